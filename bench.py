@@ -1,0 +1,193 @@
+#!/usr/bin/env python3
+"""Benchmark of the MI355X hip_solver on its headline workload.
+
+Workload (one GPU): the 8192 x 8192 FP64 dam break on [0,20]^2, flat bottom, reflecting
+walls, h = 2 for x < 10 else 1 plus 1e-3 seeded uniform noise, u = v = 0, fixed dt from
+CFL 0.45 at t = 0 (the reference's fixed-dt policy).  A "step" is one SSP-RK3 step of the
+whole grid (three fused stage launches).  Inputs are resident in HBM before timing.
+
+Prints ONE JSON line.  `value` = interior cell-updates per second / 1e9 over the timed
+steps (wall clock around the replayed steps, synchronised).  `roofline` is priced from HIP
+events recorded on the solver's stream around every stage launch in a second timed pass.
+`cpu_baseline` times the reference's own OpenMP kernels (oracle/_ref) on the host on a
+few steps of the same grid; those steps are also compared bitwise with the GPU.
+
+    python bench.py [--steps K] [--warmup W] [--n 8192] [--cpu-steps S] [--no-cpu]
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+
+REPO = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, REPO)
+sys.path.insert(0, os.path.join(REPO, "shallow-water_amd"))
+
+import numpy as np  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0            # MI355X HBM3E peak, GB/s
+BYTES_PER_CELL_STAGE = (56, 80, 80)  # algorithmic HBM bytes per interior cell, stages 1..3
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def mass(h, ng):
+    return math.fsum(h[ng:-ng, ng:-ng].ravel().tolist())
+
+
+def cpu_baseline(case, steps, dt):
+    """Time the reference's OpenMP kernels (or the port when the reference build is absent)
+    on `steps` steps of the same grid.  Returns (info dict, final state)."""
+    import oracle
+
+    gr = oracle.grid(case["nx"], case["ny"], case["ng"], case["dx"], case["dy"], g=case["g"],
+                     bc=case["bc"])
+    h, hu, hv = (case[k].copy() for k in ("h", "hu", "hv"))
+    B = case["B"].copy()
+    oracle.apply_bc(gr, B)
+    if oracle.ref_available():
+        kind, threads = "reference", oracle.ref().ref_num_threads()
+        t0 = time.perf_counter()
+        rc, _ = oracle.ref_run(gr, h, hu, hv, B, steps, dt=dt)
+        el = time.perf_counter() - t0
+    else:
+        kind, threads = "port", oracle.lib().oracle_num_threads()
+        t0 = time.perf_counter()
+        rc, _, _ = oracle.run(gr, h, hu, hv, B, steps, dt=dt)
+        el = time.perf_counter() - t0
+    if rc != 0:
+        raise RuntimeError("CPU baseline run failed")
+    cells = case["nx"] * case["ny"]
+    info = {
+        "value": cells * steps / el / 1e9, "unit": "Gcell-updates/s", "cores": threads,
+        "kind": kind,
+        "sample": f"{case['nx']}x{case['ny']} dam break, {steps} fixed-dt SSP-RK3 steps "
+                  f"({'oracle/_ref: reference omp_kernels.hpp, OpenMP' if kind == 'reference' else 'oracle port, OpenMP'}"
+                  f", g++ -O3 -ffp-contract=off), {el:.2f} s",
+    }
+    return info, (h, hu, hv)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--n", type=int, default=8192)
+    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tile-rows", type=int, default=0)
+    ap.add_argument("--event-steps", type=int, default=20)
+    args = ap.parse_args()
+
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py measures one GPU only: the multi-GPU slab decomposition is a design "
+              "(DESIGN.md), not part of this build. Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import swe_hip
+    from swe_hip import workloads as W
+
+    if swe_hip.device_count() < 1:
+        raise SystemExit("no HIP device visible")
+
+    n = args.n
+    log(f"[bench] generating dam break {n}^2")
+    case = W.dam_break(n)
+    s = swe_hip.HipSolver.from_case(case)
+    if args.tile_rows:
+        s.tile_rows = args.tile_rows
+    s.upload(case["h"], case["hu"], case["hv"], case["B"])
+    dt, max_speed = s.stable_dt(0.45)
+    m0 = mass(case["h"], case["ng"])
+    log(f"[bench] dt={dt!r} max_speed={max_speed!r} tile_rows={s.tile_rows}")
+
+    # warmup (also builds the step graph)
+    s.step(dt, max(args.warmup, 1))
+    s.sync()
+    t0 = time.perf_counter()
+    s.step(dt, args.steps)        # returns after a stream sync + non-finite check
+    el = time.perf_counter() - t0
+    cells = n * n
+    value = cells * args.steps / el / 1e9
+    ms_step = el / args.steps * 1e3
+    log(f"[bench] {args.steps} steps in {el:.3f} s: {ms_step:.3f} ms/step, {value:.2f} Gcell/s")
+
+    # per-stage event timing (direct launches, events on the solver stream)
+    stage_ms, ev_total = s.time_steps(dt, args.event_steps)
+    log(f"[bench] stage ms {stage_ms}, event total {ev_total:.3f} ms / {args.event_steps} steps")
+    h_end = s.download()[0]
+    m1 = mass(h_end, case["ng"])
+    steps_done = max(args.warmup, 1) + args.steps + args.event_steps
+    stage_bytes = [b * cells for b in BYTES_PER_CELL_STAGE]
+    dom = 1  # stage 2 kernel: 80 B/cell, the largest per launch
+    ach_dom = stage_bytes[dom] / (stage_ms[dom] * 1e-3) / 1e9
+    step_bytes = sum(stage_bytes)
+    ach_step = step_bytes / (sum(stage_ms) * 1e-3) / 1e9
+    traffic = None
+    tf = os.path.join(REPO, "profiles", "stage_kernel_dam_break_8192_traffic.json")
+    if os.path.exists(tf) and n == 8192:
+        try:
+            traffic = json.load(open(tf)).get("stage2_hbm_bytes_per_launch")
+        except Exception:
+            traffic = None
+
+    out = {
+        "metric": "FP64 Gcell-updates/s @8192^2 (dam break, SSP-RK3 MUSCL+HLL+HR)",
+        "value": round(value, 4),
+        "unit": "Gcell-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded dam break)",
+        "config": {"workload": f"dam_break_{n}x{n}_fp64_reflecting_cfl0.45", "nx": n, "ny": n,
+                   "ng": 2, "dt": dt, "parallelism": "single GPU",
+                   "tile_rows": s.tile_rows},
+        "roofline": {
+            "bound": "hbm", "kernel": "stage_kernel<2> (SSP-RK3 stage 2, 80 B/cell)",
+            "achieved": round(ach_dom, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach_dom / HBM_PEAK_GBS, 4), "traffic": traffic,
+        },
+        "roofline_step": {
+            "bytes_per_cell_update": sum(BYTES_PER_CELL_STAGE), "achieved": round(ach_step, 1),
+            "frac": round(ach_step / HBM_PEAK_GBS, 4), "stage_ms": [round(x, 4) for x in stage_ms],
+        },
+        "pct_hbm_peak": round(100.0 * value * 1e9 * 216 / (HBM_PEAK_GBS * 1e9), 2),
+        "mass_rel_change": abs(m1 - m0) / m0,
+        "steps_total_on_state": steps_done,
+    }
+    s.close()
+
+    if not args.no_cpu:
+        log(f"[bench] CPU baseline: {args.cpu_steps} steps")
+        try:
+            info, ref_state = cpu_baseline(case, args.cpu_steps, dt)
+            out["cpu_baseline"] = info
+            # same steps on the GPU, compared bitwise
+            with swe_hip.HipSolver.from_case(case) as s2:
+                s2.upload(case["h"], case["hu"], case["hv"], case["B"])
+                s2.step(dt, args.cpu_steps)
+                got = s2.download()
+            same = all(np.array_equal(a.view(np.uint64), b.view(np.uint64))
+                       for a, b in zip(got, ref_state))
+            out["cpu_parity_bitwise"] = bool(same)
+            log(f"[bench] cpu {info['value']:.4f} Gcell/s ({info['cores']} threads), bitwise={same}")
+        except MemoryError as e:
+            out["cpu_baseline"] = None
+            log(f"[bench] CPU baseline skipped: {e}")
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,137 @@
+/*
+ * swe_hip.h -- C ABI of the MI355X (gfx950) shallow-water backend, libswe_hip.so.
+ *
+ * The reference has no foreign-function interface: its backends are C++ classes selected by
+ * SolverAssembly (include/solver_assembly/solver_assembly.hpp:34-120).  The replaced
+ * single-GPU backend, FastHLLMUSCLBathyCUDASolver (include/backend/cuda/cuda_solver.hpp),
+ * keeps its device arrays and kernels private.  This header exposes the same operations as
+ * a C ABI (plain pointers and sizes, no C++ or torch types) so the host-only C++ backend
+ * class, Python (ctypes) and any other FFI can drive them.  Each entry point names the
+ * reference code it replaces.
+ *
+ * Layout.  Every array is float64 of n_total = (nx + 2*ng) * (ny + 2*ng) elements in the
+ * reference layout: row-major, k = i*(ny + 2*ng) + j, i = x index (slow), j = y index
+ * (contiguous), interior i, j in [ng, ng + n).  (omp_kernels.hpp:33-35, grid.hpp:11-12)
+ *
+ * Conventions.  Every int-returning function returns SWE_OK (0) or an swe_status code;
+ * swe_hip_last_error() gives the message for the most recent failure on that context.
+ * Host pointers are caller-owned and not retained.  Calls on one context are not
+ * thread-safe; distinct contexts are independent.  All device work of a context runs on
+ * its stream; functions that return host data synchronize that stream.
+ */
+#ifndef SWE_HIP_H
+#define SWE_HIP_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define SWE_HIP_ABI_VERSION 1
+
+typedef enum {
+    SWE_OK = 0,
+    SWE_ERR_INVALID = 1,    /* bad argument or descriptor */
+    SWE_ERR_HIP = 2,        /* HIP runtime error (message in swe_hip_last_error) */
+    SWE_ERR_NONFINITE = 3,  /* non-finite h/hu/hv after a stage (omp_kernels.hpp:322-330) */
+    SWE_ERR_CFL = 4,        /* dt exceeds the CFL limit (omp_solver.hpp:159-162) */
+    SWE_ERR_OOM = 5,        /* device allocation failed */
+    SWE_ERR_NO_DEVICE = 6   /* no usable HIP device */
+} swe_status;
+
+typedef enum {
+    SWE_BC_REFLECTING = 0,  /* reference walls (omp_solver.hpp:340-432) */
+    SWE_BC_TRANSMISSIVE = 1 /* zero-gradient copy; extension, no reference counterpart */
+} swe_bc_kind;
+
+/* Grid descriptor.  Replaces Grid (include/core/grid.hpp:3-31) + GridView
+ * (omp_kernels.hpp:22-31) + constants::g (include/constants.hpp:4).  dx = Lx/Nx. */
+typedef struct {
+    int32_t nx, ny, ng; /* interior cells per axis; ghost width (>= 2) */
+    double dx, dy;
+    double g;           /* gravity; the reference uses 9.80665 */
+    int32_t bc_kind;    /* swe_bc_kind */
+} swe_grid_desc;
+
+typedef struct swe_hip_ctx swe_hip_ctx;
+
+int swe_hip_abi_version(void);
+
+/* Status code -> static string. */
+const char *swe_hip_error_string(int code);
+
+/* Detailed message of the last failure on ctx ("" if none). */
+const char *swe_hip_last_error(const swe_hip_ctx *ctx);
+
+/* Number of visible HIP devices (0 when there is no GPU). */
+int swe_hip_device_count(void);
+
+/* Allocate a context on `device`: U, U1, U2 (3 arrays each), B, scalars.
+ * Replaces FastHLLMUSCLBathyCUDASolver::allocate_device (cuda_solver.hpp:233-252). */
+int swe_hip_create(const swe_grid_desc *desc, int device, swe_hip_ctx **out);
+
+/* Free all device memory of ctx (cuda_solver.hpp:254-273).  NULL is a no-op. */
+void swe_hip_destroy(swe_hip_ctx *ctx);
+
+/* Upload state and bathymetry from host (n_total each; ghosts are ignored and rebuilt:
+ * B gets the scalar ghost fill of omp_solver.hpp:395-432 as at solver construction).
+ * Replaces upload_initial_state_to_device (cuda_solver.hpp:277-286). */
+int swe_hip_upload(swe_hip_ctx *ctx, const double *h, const double *hu, const double *hv,
+                   const double *b);
+
+/* Download the current state with its ghost cells filled as after apply_reflecting_bc.
+ * Replaces download_state_from_device (cuda_solver.hpp:288-296). */
+int swe_hip_download(swe_hip_ctx *ctx, double *h, double *hu, double *hv);
+
+/* Same as upload/download with device pointers (e.g. torch tensors on ctx's device);
+ * the copies are ordered on ctx's stream. */
+int swe_hip_upload_device(swe_hip_ctx *ctx, const double *h, const double *hu, const double *hv,
+                          const double *b);
+int swe_hip_download_device(swe_hip_ctx *ctx, double *h, double *hu, double *hv);
+
+/* compute_stable_dt of the current state (omp_kernels.hpp:288-307).  Either output may be
+ * NULL. */
+int swe_hip_stable_dt(swe_hip_ctx *ctx, double cfl, double *dt_out, double *max_speed_out);
+
+/* Advance nsteps SSP-RK3 steps with a fixed dt: the reference's run loop
+ * (omp_solver.hpp:185-214 / rk3_step :250-338; cuda_solver.hpp:195-227 / :400-417).
+ * Non-finite states are detected per stage on the device and reported once per call
+ * (SWE_ERR_NONFINITE) instead of at the first bad stage. */
+int swe_hip_step(swe_hip_ctx *ctx, double dt, int64_t nsteps);
+
+/* Advance nsteps steps, each with dt = compute_stable_dt(U^n, cfl) evaluated on the device
+ * (extension: the reference fixes dt = end_time/steps, omp_solver.hpp:99).  dt_hist (may be
+ * NULL) receives the nsteps dt values used. */
+int swe_hip_step_adaptive(swe_hip_ctx *ctx, double cfl, int64_t nsteps, double *dt_hist);
+
+/* compute_rhs of the current state (omp_solver.hpp:220-248): rhs arrays of n_total on the
+ * host, interior written, ghosts 0 (as the reference's zero-initialized vectors). */
+int swe_hip_rhs(swe_hip_ctx *ctx, double *rhs_h, double *rhs_hu, double *rhs_hv);
+
+/* Launch shape of the fused stage kernel: interior rows marched per wave (>= 1).
+ * Default picked by swe_hip_create from the grid size. */
+int swe_hip_set_tile_rows(swe_hip_ctx *ctx, int tile_rows);
+int swe_hip_get_tile_rows(const swe_hip_ctx *ctx);
+
+/* Use a caller stream (hipStream_t passed as void*; NULL = the context's own stream). */
+int swe_hip_set_stream(swe_hip_ctx *ctx, void *stream);
+
+/* Replay-mode switch: 1 = capture one step in a hipGraph and replay it (default),
+ * 0 = launch kernels directly. */
+int swe_hip_set_graph(swe_hip_ctx *ctx, int enabled);
+
+/* Timing helper for benchmarks: runs nsteps fixed-dt steps with direct launches and HIP
+ * events recorded on ctx's stream around every stage launch; stage_ms[0..2] receive the
+ * mean duration of stages 1..3 and *total_ms the event time of the whole loop. */
+int swe_hip_time_steps(swe_hip_ctx *ctx, double dt, int64_t nsteps, double *stage_ms,
+                       double *total_ms);
+
+/* Wait for all work on ctx's stream. */
+int swe_hip_sync(swe_hip_ctx *ctx);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* SWE_HIP_H */

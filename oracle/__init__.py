@@ -1,0 +1,212 @@
+"""ctypes bindings for the CPU oracle.
+
+TEST INFRASTRUCTURE ONLY.  Only tests/, ``__graft_entry__.smoke()`` and bench.py's
+``cpu_baseline`` leg may import this package, and only as the checker (or as the timed
+CPU baseline).  The product path (``swe_hip``) never imports it.
+
+Two libraries:
+
+* ``liboracle.so`` -- this build's own C restatement (oracle/swe_oracle.c) of the
+  reference's ``include/backend/openmp/omp_kernels.hpp`` plus the ``rk3_step`` driver
+  of ``include/backend/openmp/omp_solver.hpp:250-338``.  g is a run-time parameter.
+* ``_ref/libswe_ref.so`` -- the reference's own header-only kernels compiled where they
+  lie (oracle/ref_driver.cpp + ``make -C oracle ref``).  Present only where the
+  reference tree was available at build time (the .so itself travels with the tree).
+  g is fixed at the reference's 9.80665.
+
+Arrays are float64, C-contiguous, shape ``(nx + 2*ng, ny + 2*ng)`` in the reference
+layout (row-major, ``k = i*(ny+2*ng) + j``, i = x index, j = y index).
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ORACLE_SO = os.path.join(HERE, "liboracle.so")
+REF_SO = os.path.join(HERE, "_ref", "libswe_ref.so")
+
+G_REF = 9.80665  # include/constants.hpp:4
+BC_REFLECTING = 0
+BC_TRANSMISSIVE = 1
+
+_dp = C.POINTER(C.c_double)
+
+
+class OracleGrid(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("ng", C.c_int32),
+        ("dx", C.c_double), ("dy", C.c_double), ("g", C.c_double),
+        ("bc_kind", C.c_int32),
+    ]
+
+
+def build(ref: bool | None = None) -> None:
+    """Compile liboracle.so (and the reference harness when the reference tree exists)."""
+    subprocess.run(["make", "-s", "-C", HERE, "-B", "liboracle.so"], check=True)
+    ref_root = os.environ.get("SWE_REFERENCE_ROOT", "/root/reference")
+    if ref is None:
+        ref = os.path.isdir(os.path.join(ref_root, "include", "backend", "openmp"))
+    if ref:
+        subprocess.run(["make", "-s", "-C", HERE, "-B", "ref", f"REF={ref_root}"], check=True)
+
+
+_lib = None
+_ref = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(ORACLE_SO):
+            subprocess.run(["make", "-s", "-C", HERE, "liboracle.so"], check=True)
+        L = C.CDLL(ORACLE_SO)
+        L.oracle_minmod.restype = C.c_double
+        L.oracle_minmod.argtypes = [C.c_double, C.c_double]
+        for nm in ("oracle_hll_x", "oracle_hll_y"):
+            getattr(L, nm).restype = None
+            getattr(L, nm).argtypes = [C.c_double] * 7 + [_dp]
+        for nm in ("oracle_x_fluxes", "oracle_y_fluxes"):
+            getattr(L, nm).restype = None
+            getattr(L, nm).argtypes = [C.POINTER(OracleGrid)] + [_dp] * 10
+        L.oracle_rhs.restype = C.c_int
+        L.oracle_rhs.argtypes = [C.POINTER(OracleGrid)] + [_dp] * 7
+        L.oracle_stable_dt.restype = C.c_double
+        L.oracle_stable_dt.argtypes = [C.POINTER(OracleGrid), _dp, _dp, _dp, C.c_double, _dp]
+        L.oracle_enforce_positivity.restype = C.c_int
+        L.oracle_enforce_positivity.argtypes = [C.POINTER(OracleGrid), _dp, _dp, _dp]
+        L.oracle_apply_bc.restype = None
+        L.oracle_apply_bc.argtypes = [C.POINTER(OracleGrid), _dp, _dp, _dp]
+        L.oracle_run.restype = C.c_int
+        L.oracle_run.argtypes = [C.POINTER(OracleGrid), _dp, _dp, _dp, _dp, C.c_int, C.c_double,
+                                 C.c_long, _dp, C.POINTER(C.c_long)]
+        L.oracle_num_threads.restype = C.c_int
+        _lib = L
+    return _lib
+
+
+def ref_available() -> bool:
+    return os.path.exists(REF_SO)
+
+
+def ref():
+    """The reference harness, or None when it was not built."""
+    global _ref
+    if _ref is None and ref_available():
+        R = C.CDLL(REF_SO)
+        R.ref_g.restype = C.c_double
+        R.ref_num_threads.restype = C.c_int
+        for nm in ("ref_minmod", "hpc_minmod"):
+            getattr(R, nm).restype = C.c_double
+            getattr(R, nm).argtypes = [C.c_double, C.c_double]
+        for nm in ("ref_hll_x", "ref_hll_y"):
+            getattr(R, nm).restype = None
+            getattr(R, nm).argtypes = [C.c_double] * 6 + [_dp]
+        grid = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]
+        for nm in ("ref_faces_and_rhs", "hpc_faces_and_rhs"):
+            getattr(R, nm).restype = None
+            getattr(R, nm).argtypes = grid + [_dp] * 4 + [C.POINTER(_dp), C.POINTER(_dp)]
+        R.ref_stable_dt.restype = C.c_double
+        R.ref_stable_dt.argtypes = grid + [_dp] * 3 + [C.c_double]
+        R.ref_enforce_positivity.restype = C.c_int
+        R.ref_enforce_positivity.argtypes = grid + [_dp] * 3
+        R.ref_reflect.restype = None
+        R.ref_reflect.argtypes = grid + [_dp] * 3
+        R.ref_run.restype = C.c_int
+        R.ref_run.argtypes = grid + [_dp] * 4 + [C.c_int, C.c_double, C.c_long, _dp]
+        _ref = R
+    return _ref
+
+
+def _p(a: np.ndarray):
+    assert a.dtype == np.float64 and a.flags.c_contiguous
+    return a.ctypes.data_as(_dp)
+
+
+def grid(nx, ny, ng=2, dx=None, dy=None, g=G_REF, bc=BC_REFLECTING, lx=1.0, ly=1.0):
+    dx = lx / nx if dx is None else dx
+    dy = ly / ny if dy is None else dy
+    return OracleGrid(nx, ny, ng, dx, dy, g, bc)
+
+
+# ---------------------------------------------------------------- oracle wrappers
+def minmod(a: float, b: float) -> float:
+    return lib().oracle_minmod(a, b)
+
+
+def hll(direction: str, hL, huL, hvL, hR, huR, hvR, g=G_REF):
+    out = (C.c_double * 3)()
+    fn = lib().oracle_hll_x if direction == "x" else lib().oracle_hll_y
+    fn(g, hL, huL, hvL, hR, huR, hvR, out)
+    return tuple(out)
+
+
+def apply_bc(gr: OracleGrid, h, hu=None, hv=None):
+    lib().oracle_apply_bc(C.byref(gr), _p(h), _p(hu) if hu is not None else None,
+                          _p(hv) if hv is not None else None)
+
+
+def faces(gr: OracleGrid, h, hu, hv, B):
+    """The 12 face arrays fxm(h,hu,hv) fxp(..) fym(..) fyp(..) of the current arrays (no
+    ghost fill)."""
+    f = [np.zeros_like(h) for _ in range(12)]
+    lib().oracle_x_fluxes(C.byref(gr), _p(h), _p(hu), _p(hv), _p(B), *(_p(a) for a in f[:6]))
+    lib().oracle_y_fluxes(C.byref(gr), _p(h), _p(hu), _p(hv), _p(B), *(_p(a) for a in f[6:]))
+    return f
+
+
+def rhs(gr: OracleGrid, h, hu, hv, B):
+    """Ghost fill (in place) then compute_rhs; returns (rhs_h, rhs_hu, rhs_hv)."""
+    out = [np.zeros_like(h) for _ in range(3)]
+    rc = lib().oracle_rhs(C.byref(gr), _p(h), _p(hu), _p(hv), _p(B), *(_p(o) for o in out))
+    if rc != 0:
+        raise MemoryError("oracle_rhs allocation failed")
+    return out
+
+
+def stable_dt(gr: OracleGrid, h, hu, hv, cfl):
+    ms = C.c_double()
+    dt = lib().oracle_stable_dt(C.byref(gr), _p(h), _p(hu), _p(hv), cfl, C.byref(ms))
+    return dt, ms.value
+
+
+def enforce_positivity(gr: OracleGrid, h, hu, hv) -> int:
+    return lib().oracle_enforce_positivity(C.byref(gr), _p(h), _p(hu), _p(hv))
+
+
+def run(gr: OracleGrid, h, hu, hv, B, nsteps, dt=None, cfl=None):
+    """Advance in place.  Exactly one of dt (fixed) or cfl (adaptive per step).
+
+    Returns (rc, steps_done, dt_hist)."""
+    assert (dt is None) != (cfl is None)
+    hist = np.zeros(max(nsteps, 1))
+    done = C.c_long()
+    mode, val = (0, dt) if dt is not None else (1, cfl)
+    rc = lib().oracle_run(C.byref(gr), _p(h), _p(hu), _p(hv), _p(B), mode, val, nsteps,
+                          _p(hist), C.byref(done))
+    if rc < 0:
+        raise MemoryError("oracle_run allocation failed")
+    return rc, done.value, hist[:nsteps]
+
+
+# ---------------------------------------------------------------- reference harness
+def ref_faces_and_rhs(gr: OracleGrid, h, hu, hv, B, hpc=False):
+    f = [np.zeros_like(h) for _ in range(12)]
+    r = [np.zeros_like(h) for _ in range(3)]
+    fp = (_dp * 12)(*[_p(a) for a in f])
+    rp = (_dp * 3)(*[_p(a) for a in r])
+    fn = ref().hpc_faces_and_rhs if hpc else ref().ref_faces_and_rhs
+    fn(gr.nx, gr.ny, gr.ng, gr.dx, gr.dy, _p(h), _p(hu), _p(hv), _p(B), fp, rp)
+    return f, r
+
+
+def ref_run(gr: OracleGrid, h, hu, hv, B, nsteps, dt=None, cfl=None):
+    assert (dt is None) != (cfl is None)
+    hist = np.zeros(max(nsteps, 1))
+    mode, val = (0, dt) if dt is not None else (1, cfl)
+    rc = ref().ref_run(gr.nx, gr.ny, gr.ng, gr.dx, gr.dy, _p(h), _p(hu), _p(hv), _p(B), mode, val,
+                       nsteps, _p(hist))
+    return rc, hist[:nsteps]

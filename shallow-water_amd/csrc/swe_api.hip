@@ -1,0 +1,474 @@
+// swe_api.hip -- C ABI of libswe_hip.so (declared in include/swe_hip.h).
+//
+// A context owns one grid's device arrays on one device and one stream:
+//   U = (h, hu, hv), U1, U2 (stage buffers), B, and a few scalars (dt, max speed),
+// the per-block partial maxima of the CFL reduction and the non-finite flag.
+// A step is three fused stage launches U -> U1 -> U2 -> U (stage 3 writes U in place: it
+// reads U^n only at the cell it writes).  With graphs enabled one step is captured once
+// and replayed.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/swe_hip.h"
+#include "swe_internal.h"
+
+using swe::GridDev;
+using swe::StageArgs;
+
+struct swe_hip_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    GridDev G{};
+    int bc = 0;
+    size_t n_total = 0;
+    double *U[3] = {nullptr, nullptr, nullptr};
+    double *U1[3] = {nullptr, nullptr, nullptr};
+    double *U2[3] = {nullptr, nullptr, nullptr};
+    double *B = nullptr;
+    double *scal = nullptr;      // [0] dt, [1] max speed
+    double *partial = nullptr;   // per-block maxima
+    int partial_cap = 0;
+    int *flag = nullptr;
+    long long *counter = nullptr;
+    double *hist = nullptr;
+    long long hist_cap = 0;
+    int tile_rows = 64;
+    bool use_graph = true;
+    // cached graph of one step: key = (mode, tile_rows, stream, hist pointer)
+    hipGraphExec_t graph = nullptr;
+    int graph_mode = -1;
+    int graph_tile = -1;
+    hipStream_t graph_stream = nullptr;
+    double graph_cfl = 0.0;
+    std::string err;
+};
+
+namespace {
+
+int fail(swe_hip_ctx *c, int code, const std::string &msg) {
+    if (c) c->err = msg;
+    return code;
+}
+
+#define SWE_CHECK(ctx, call)                                                                 \
+    do {                                                                                     \
+        const hipError_t e__ = (call);                                                       \
+        if (e__ != hipSuccess) {                                                             \
+            return fail(ctx, SWE_ERR_HIP,                                                    \
+                        std::string(#call) + ": " + hipGetErrorString(e__) + " (" __FILE__ ":" + \
+                            std::to_string(__LINE__) + ")");                                 \
+        }                                                                                    \
+    } while (0)
+
+int default_tile_rows(const GridDev &G) {
+    // Enough waves to fill 256 CUs several times over, long enough marches to amortise
+    // the 4-row halo of each segment.
+    const long strips = swe::strips_for(G.ny);
+    int tr = 128;
+    while (tr > 16 && strips * ((G.nx + tr - 1) / tr) < 8L * 256 * 4) tr /= 2;
+    return tr;
+}
+
+void drop_graph(swe_hip_ctx *c) {
+    if (c->graph) hipGraphExecDestroy(c->graph);
+    c->graph = nullptr;
+    c->graph_mode = -1;
+}
+
+int ensure_partial(swe_hip_ctx *c, int need) {
+    if (need <= c->partial_cap) return SWE_OK;
+    drop_graph(c);  // a captured step bakes the partial-max pointer
+    if (c->partial) hipFree(c->partial);
+    c->partial = nullptr;
+    c->partial_cap = 0;
+    SWE_CHECK(c, hipMalloc(&c->partial, sizeof(double) * (size_t)need));
+    c->partial_cap = need;
+    return SWE_OK;
+}
+
+StageArgs stage_args(swe_hip_ctx *c, int stage) {
+    StageArgs a{};
+    double *const *in = stage == 1 || stage == 0 ? c->U : (stage == 2 ? c->U1 : c->U2);
+    // stage 0 (rhs only) writes into the U1 buffers, never over its own input
+    double *const *out = (stage == 0 || stage == 1) ? c->U1 : (stage == 2 ? c->U2 : c->U);
+    a.in_h = in[0];
+    a.in_hu = in[1];
+    a.in_hv = in[2];
+    a.u0_h = c->U[0];
+    a.u0_hu = c->U[1];
+    a.u0_hv = c->U[2];
+    a.B = c->B;
+    a.out_h = out[0];
+    a.out_hu = out[1];
+    a.out_hv = out[2];
+    a.dt = c->scal;
+    a.partial = c->partial;
+    a.flag = c->flag;
+    a.tile_rows = c->tile_rows;
+    a.nstrips = swe::strips_for(c->G.ny);
+    return a;
+}
+
+// Enqueue one step.  mode 0: fixed dt (scal[0] holds it); mode 1: adaptive, stage 3 reduces
+// the new max speed and a finalize kernel writes the next dt into scal[0] (and the history).
+int enqueue_step(swe_hip_ctx *c, int mode, double cfl) {
+    const bool speed = mode == 1;
+    for (int stage = 1; stage <= 3; ++stage) {
+        StageArgs a = stage_args(c, stage);
+        dim3 grid;
+        SWE_CHECK(c, swe::launch_stage(stage, c->bc, speed && stage == 3, c->G, a, c->stream,
+                                       &grid));
+        if (speed && stage == 3) {
+            SWE_CHECK(c, swe::launch_finalize_dt(c->partial, (int)(grid.x * grid.y), cfl, c->G.dx,
+                                                 c->G.dy, c->scal, c->hist, c->counter,
+                                                 c->hist_cap, c->stream));
+        }
+    }
+    return SWE_OK;
+}
+
+int run_steps(swe_hip_ctx *c, int mode, double cfl, int64_t nsteps) {
+    if (!c->use_graph || nsteps < 2) {
+        for (int64_t s = 0; s < nsteps; ++s) {
+            const int rc = enqueue_step(c, mode, cfl);
+            if (rc) return rc;
+        }
+        return SWE_OK;
+    }
+    if (!c->graph || c->graph_mode != mode || c->graph_tile != c->tile_rows ||
+        c->graph_stream != c->stream || (mode == 1 && c->graph_cfl != cfl)) {
+        drop_graph(c);
+        hipGraph_t g = nullptr;
+        SWE_CHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int rc = enqueue_step(c, mode, cfl);
+        hipGraph_t captured = nullptr;
+        const hipError_t e = hipStreamEndCapture(c->stream, &captured);
+        if (rc) {
+            if (captured) hipGraphDestroy(captured);
+            return rc;
+        }
+        SWE_CHECK(c, e);
+        g = captured;
+        const hipError_t ei = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
+        hipGraphDestroy(g);
+        SWE_CHECK(c, ei);
+        c->graph_mode = mode;
+        c->graph_tile = c->tile_rows;
+        c->graph_stream = c->stream;
+        c->graph_cfl = cfl;
+    }
+    for (int64_t s = 0; s < nsteps; ++s) SWE_CHECK(c, hipGraphLaunch(c->graph, c->stream));
+    return SWE_OK;
+}
+
+int check_flag(swe_hip_ctx *c) {
+    int flag = 0;
+    SWE_CHECK(c, hipMemcpyAsync(&flag, c->flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    SWE_CHECK(c, hipStreamSynchronize(c->stream));
+    if (flag) return fail(c, SWE_ERR_NONFINITE, "non-finite state detected in HIP solver");
+    return SWE_OK;
+}
+
+int upload_common(swe_hip_ctx *c, const double *h, const double *hu, const double *hv,
+                  const double *b, hipMemcpyKind kind) {
+    const size_t bytes = c->n_total * sizeof(double);
+    SWE_CHECK(c, hipMemcpyAsync(c->U[0], h, bytes, kind, c->stream));
+    SWE_CHECK(c, hipMemcpyAsync(c->U[1], hu, bytes, kind, c->stream));
+    SWE_CHECK(c, hipMemcpyAsync(c->U[2], hv, bytes, kind, c->stream));
+    SWE_CHECK(c, hipMemcpyAsync(c->B, b, bytes, kind, c->stream));
+    // B ghosts as at solver construction (omp_solver.hpp:119); state ghosts rebuilt too.
+    SWE_CHECK(c, swe::launch_ghost_fill(c->bc, c->G, c->B, nullptr, nullptr, c->stream));
+    SWE_CHECK(c, swe::launch_ghost_fill(c->bc, c->G, c->U[0], c->U[1], c->U[2], c->stream));
+    SWE_CHECK(c, hipStreamSynchronize(c->stream));
+    return SWE_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int swe_hip_abi_version(void) { return SWE_HIP_ABI_VERSION; }
+
+const char *swe_hip_error_string(int code) {
+    switch (code) {
+        case SWE_OK: return "ok";
+        case SWE_ERR_INVALID: return "invalid argument";
+        case SWE_ERR_HIP: return "HIP runtime error";
+        case SWE_ERR_NONFINITE: return "non-finite state detected";
+        case SWE_ERR_CFL: return "time step exceeds the CFL limit";
+        case SWE_ERR_OOM: return "device allocation failed";
+        case SWE_ERR_NO_DEVICE: return "no HIP device";
+        default: return "unknown error";
+    }
+}
+
+const char *swe_hip_last_error(const swe_hip_ctx *ctx) { return ctx ? ctx->err.c_str() : ""; }
+
+int swe_hip_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+int swe_hip_create(const swe_grid_desc *d, int device, swe_hip_ctx **out) {
+    if (!d || !out) return SWE_ERR_INVALID;
+    *out = nullptr;
+    if (d->ng < 2 || d->nx < d->ng || d->ny < d->ng || !(d->dx > 0.0) || !(d->dy > 0.0) ||
+        !(d->g > 0.0) || (d->bc_kind != SWE_BC_REFLECTING && d->bc_kind != SWE_BC_TRANSMISSIVE))
+        return SWE_ERR_INVALID;
+    const long long nxt = (long long)d->nx + 2LL * d->ng, nyt = (long long)d->ny + 2LL * d->ng;
+    if (nxt * nyt >= (1LL << 31)) return SWE_ERR_INVALID;  // 32-bit cell indices
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return SWE_ERR_NO_DEVICE;
+    if (device < 0 || device >= ndev) return SWE_ERR_INVALID;
+
+    auto *c = new swe_hip_ctx();
+    c->device = device;
+    c->bc = d->bc_kind;
+    c->G.nx = d->nx;
+    c->G.ny = d->ny;
+    c->G.ng = d->ng;
+    c->G.stride = (int)nyt;
+    c->G.dx = d->dx;
+    c->G.dy = d->dy;
+    c->G.inv_dx = 1.0 / d->dx;
+    c->G.inv_dy = 1.0 / d->dy;
+    c->G.g = d->g;
+    c->G.half_g = 0.5 * d->g;
+    c->n_total = (size_t)(nxt * nyt);
+    c->tile_rows = default_tile_rows(c->G);
+
+    auto bad = [&](int code, const std::string &m) {
+        swe_hip_destroy(c);
+        (void)m;
+        return code;
+    };
+    if (hipSetDevice(device) != hipSuccess) return bad(SWE_ERR_HIP, "hipSetDevice");
+    if (hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking) != hipSuccess)
+        return bad(SWE_ERR_HIP, "stream");
+    c->stream = c->own_stream;
+    const size_t bytes = c->n_total * sizeof(double);
+    double **arrays[] = {&c->U[0], &c->U[1], &c->U[2], &c->U1[0], &c->U1[1],
+                         &c->U1[2], &c->U2[0], &c->U2[1], &c->U2[2], &c->B};
+    for (double **a : arrays)
+        if (hipMalloc(a, bytes) != hipSuccess) return bad(SWE_ERR_OOM, "hipMalloc");
+    if (hipMalloc(&c->scal, 4 * sizeof(double)) != hipSuccess) return bad(SWE_ERR_OOM, "scal");
+    if (hipMalloc(&c->flag, sizeof(int)) != hipSuccess) return bad(SWE_ERR_OOM, "flag");
+    if (hipMalloc(&c->counter, sizeof(long long)) != hipSuccess) return bad(SWE_ERR_OOM, "ctr");
+    for (double **a : arrays)
+        if (swe::launch_fill_double(*a, 0.0, (long)c->n_total, c->stream) != hipSuccess)
+            return bad(SWE_ERR_HIP, "fill");
+    if (swe::launch_fill_double(c->scal, 0.0, 4, c->stream) != hipSuccess ||
+        swe::launch_fill_int(c->flag, 0, 1, c->stream) != hipSuccess)
+        return bad(SWE_ERR_HIP, "fill");
+    if (hipStreamSynchronize(c->stream) != hipSuccess) return bad(SWE_ERR_HIP, "sync");
+    *out = c;
+    return SWE_OK;
+}
+
+void swe_hip_destroy(swe_hip_ctx *c) {
+    if (!c) return;
+    hipSetDevice(c->device);
+    if (c->stream) hipStreamSynchronize(c->stream);
+    drop_graph(c);
+    for (int i = 0; i < 3; ++i) {
+        hipFree(c->U[i]);
+        hipFree(c->U1[i]);
+        hipFree(c->U2[i]);
+    }
+    hipFree(c->B);
+    hipFree(c->scal);
+    hipFree(c->partial);
+    hipFree(c->flag);
+    hipFree(c->counter);
+    hipFree(c->hist);
+    if (c->own_stream) hipStreamDestroy(c->own_stream);
+    delete c;
+}
+
+int swe_hip_upload(swe_hip_ctx *c, const double *h, const double *hu, const double *hv,
+                   const double *b) {
+    if (!c || !h || !hu || !hv || !b) return SWE_ERR_INVALID;
+    SWE_CHECK(c, hipSetDevice(c->device));
+    return upload_common(c, h, hu, hv, b, hipMemcpyHostToDevice);
+}
+
+int swe_hip_upload_device(swe_hip_ctx *c, const double *h, const double *hu, const double *hv,
+                          const double *b) {
+    if (!c || !h || !hu || !hv || !b) return SWE_ERR_INVALID;
+    SWE_CHECK(c, hipSetDevice(c->device));
+    return upload_common(c, h, hu, hv, b, hipMemcpyDeviceToDevice);
+}
+
+static int download_common(swe_hip_ctx *c, double *h, double *hu, double *hv,
+                           hipMemcpyKind kind) {
+    const size_t bytes = c->n_total * sizeof(double);
+    SWE_CHECK(c, swe::launch_ghost_fill(c->bc, c->G, c->U[0], c->U[1], c->U[2], c->stream));
+    SWE_CHECK(c, hipMemcpyAsync(h, c->U[0], bytes, kind, c->stream));
+    SWE_CHECK(c, hipMemcpyAsync(hu, c->U[1], bytes, kind, c->stream));
+    SWE_CHECK(c, hipMemcpyAsync(hv, c->U[2], bytes, kind, c->stream));
+    SWE_CHECK(c, hipStreamSynchronize(c->stream));
+    return SWE_OK;
+}
+
+int swe_hip_download(swe_hip_ctx *c, double *h, double *hu, double *hv) {
+    if (!c || !h || !hu || !hv) return SWE_ERR_INVALID;
+    SWE_CHECK(c, hipSetDevice(c->device));
+    return download_common(c, h, hu, hv, hipMemcpyDeviceToHost);
+}
+
+int swe_hip_download_device(swe_hip_ctx *c, double *h, double *hu, double *hv) {
+    if (!c || !h || !hu || !hv) return SWE_ERR_INVALID;
+    SWE_CHECK(c, hipSetDevice(c->device));
+    return download_common(c, h, hu, hv, hipMemcpyDeviceToDevice);
+}
+
+int swe_hip_stable_dt(swe_hip_ctx *c, double cfl, double *dt_out, double *ms_out) {
+    if (!c) return SWE_ERR_INVALID;
+    SWE_CHECK(c, hipSetDevice(c->device));
+    const int blocks = 1024;
+    int rc = ensure_partial(c, blocks);
+    if (rc) return rc;
+    SWE_CHECK(c, swe::launch_speed_reduce(c->G, c->U[0], c->U[1], c->U[2], c->partial, blocks,
+                                          c->stream));
+    SWE_CHECK(c, swe::launch_finalize_dt(c->partial, blocks, cfl, c->G.dx, c->G.dy, c->scal,
+                                         nullptr, nullptr, 0, c->stream));
+    double v[2];
+    SWE_CHECK(c, hipMemcpyAsync(v, c->scal, sizeof(v), hipMemcpyDeviceToHost, c->stream));
+    SWE_CHECK(c, hipStreamSynchronize(c->stream));
+    if (dt_out) *dt_out = v[0];
+    if (ms_out) *ms_out = v[1];
+    return SWE_OK;
+}
+
+int swe_hip_step(swe_hip_ctx *c, double dt, int64_t nsteps) {
+    if (!c || nsteps < 0 || !(dt > 0.0)) return SWE_ERR_INVALID;
+    SWE_CHECK(c, hipSetDevice(c->device));
+    SWE_CHECK(c, swe::launch_set_scalar(c->scal, dt, c->stream));
+    SWE_CHECK(c, swe::launch_fill_int(c->flag, 0, 1, c->stream));
+    const int rc = run_steps(c, 0, 0.0, nsteps);
+    if (rc) return rc;
+    return check_flag(c);
+}
+
+int swe_hip_step_adaptive(swe_hip_ctx *c, double cfl, int64_t nsteps, double *dt_hist) {
+    if (!c || nsteps < 0 || !(cfl > 0.0)) return SWE_ERR_INVALID;
+    SWE_CHECK(c, hipSetDevice(c->device));
+    const dim3 grid = swe::stage_grid(c->G, c->tile_rows);
+    int rc = ensure_partial(c, (int)std::max<unsigned>(1024u, grid.x * grid.y));
+    if (rc) return rc;
+    if (c->hist_cap < nsteps + 1) {
+        drop_graph(c);  // the graph bakes the history pointer
+        if (c->hist) hipFree(c->hist);
+        c->hist = nullptr;
+        c->hist_cap = 0;
+        SWE_CHECK(c, hipMalloc(&c->hist, sizeof(double) * (size_t)(nsteps + 1)));
+        c->hist_cap = nsteps + 1;
+    }
+    SWE_CHECK(c, swe::launch_fill_int(reinterpret_cast<int *>(c->counter), 0, 2, c->stream));
+    SWE_CHECK(c, swe::launch_fill_int(c->flag, 0, 1, c->stream));
+    // dt of the first step from U^0
+    SWE_CHECK(c, swe::launch_speed_reduce(c->G, c->U[0], c->U[1], c->U[2], c->partial, 1024,
+                                          c->stream));
+    SWE_CHECK(c, swe::launch_finalize_dt(c->partial, 1024, cfl, c->G.dx, c->G.dy, c->scal,
+                                         c->hist, c->counter, c->hist_cap, c->stream));
+    rc = run_steps(c, 1, cfl, nsteps);
+    if (rc) return rc;
+    if (dt_hist && nsteps > 0)
+        SWE_CHECK(c, hipMemcpyAsync(dt_hist, c->hist, sizeof(double) * (size_t)nsteps,
+                                    hipMemcpyDeviceToHost, c->stream));
+    return check_flag(c);
+}
+
+int swe_hip_rhs(swe_hip_ctx *c, double *rh, double *rhu, double *rhv) {
+    if (!c || !rh || !rhu || !rhv) return SWE_ERR_INVALID;
+    SWE_CHECK(c, hipSetDevice(c->device));
+    // rhs goes to the U1 buffers (zeroed first so ghosts read back as 0)
+    for (int i = 0; i < 3; ++i)
+        SWE_CHECK(c, swe::launch_fill_double(c->U1[i], 0.0, (long)c->n_total, c->stream));
+    StageArgs a = stage_args(c, 0);
+    SWE_CHECK(c, swe::launch_stage(0, c->bc, false, c->G, a, c->stream, nullptr));
+    const size_t bytes = c->n_total * sizeof(double);
+    SWE_CHECK(c, hipMemcpyAsync(rh, c->U1[0], bytes, hipMemcpyDeviceToHost, c->stream));
+    SWE_CHECK(c, hipMemcpyAsync(rhu, c->U1[1], bytes, hipMemcpyDeviceToHost, c->stream));
+    SWE_CHECK(c, hipMemcpyAsync(rhv, c->U1[2], bytes, hipMemcpyDeviceToHost, c->stream));
+    SWE_CHECK(c, hipStreamSynchronize(c->stream));
+    return SWE_OK;
+}
+
+int swe_hip_set_tile_rows(swe_hip_ctx *c, int tile_rows) {
+    if (!c || tile_rows < 1) return SWE_ERR_INVALID;
+    c->tile_rows = tile_rows;
+    return SWE_OK;
+}
+
+int swe_hip_get_tile_rows(const swe_hip_ctx *c) { return c ? c->tile_rows : -1; }
+
+int swe_hip_set_stream(swe_hip_ctx *c, void *stream) {
+    if (!c) return SWE_ERR_INVALID;
+    c->stream = stream ? (hipStream_t)stream : c->own_stream;
+    return SWE_OK;
+}
+
+int swe_hip_set_graph(swe_hip_ctx *c, int enabled) {
+    if (!c) return SWE_ERR_INVALID;
+    c->use_graph = enabled != 0;
+    if (!c->use_graph) drop_graph(c);
+    return SWE_OK;
+}
+
+int swe_hip_time_steps(swe_hip_ctx *c, double dt, int64_t nsteps, double *stage_ms,
+                       double *total_ms) {
+    if (!c || nsteps < 1 || !(dt > 0.0)) return SWE_ERR_INVALID;
+    SWE_CHECK(c, hipSetDevice(c->device));
+    SWE_CHECK(c, swe::launch_set_scalar(c->scal, dt, c->stream));
+    SWE_CHECK(c, swe::launch_fill_int(c->flag, 0, 1, c->stream));
+    std::vector<hipEvent_t> ev((size_t)nsteps * 3 + 1);
+    for (auto &e : ev) SWE_CHECK(c, hipEventCreate(&e));
+    int rc = SWE_OK;
+    SWE_CHECK(c, hipEventRecord(ev[0], c->stream));
+    for (int64_t s = 0; s < nsteps && rc == SWE_OK; ++s) {
+        for (int stage = 1; stage <= 3; ++stage) {
+            StageArgs a = stage_args(c, stage);
+            const hipError_t e = swe::launch_stage(stage, c->bc, false, c->G, a, c->stream, nullptr);
+            if (e != hipSuccess) {
+                rc = fail(c, SWE_ERR_HIP, std::string("launch_stage: ") + hipGetErrorString(e));
+                break;
+            }
+            SWE_CHECK(c, hipEventRecord(ev[(size_t)s * 3 + stage], c->stream));
+        }
+    }
+    SWE_CHECK(c, hipStreamSynchronize(c->stream));
+    if (rc == SWE_OK) {
+        double acc[3] = {0, 0, 0};
+        for (int64_t s = 0; s < nsteps; ++s)
+            for (int stage = 1; stage <= 3; ++stage) {
+                float ms = 0.f;
+                SWE_CHECK(c, hipEventElapsedTime(&ms, ev[(size_t)s * 3 + stage - 1],
+                                                 ev[(size_t)s * 3 + stage]));
+                acc[stage - 1] += ms;
+            }
+        float tot = 0.f;
+        SWE_CHECK(c, hipEventElapsedTime(&tot, ev[0], ev[(size_t)nsteps * 3]));
+        if (stage_ms)
+            for (int i = 0; i < 3; ++i) stage_ms[i] = acc[i] / (double)nsteps;
+        if (total_ms) *total_ms = tot;
+    }
+    for (auto &e : ev) hipEventDestroy(e);
+    if (rc) return rc;
+    return check_flag(c);
+}
+
+int swe_hip_sync(swe_hip_ctx *c) {
+    if (!c) return SWE_ERR_INVALID;
+    SWE_CHECK(c, hipStreamSynchronize(c->stream));
+    return SWE_OK;
+}
+
+}  // extern "C"

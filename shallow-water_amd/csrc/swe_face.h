@@ -1,0 +1,155 @@
+// swe_face.h -- per-face arithmetic of the MUSCL + hydrostatic reconstruction + HLL scheme,
+// written for the FP64 vector ALUs of gfx950.
+//
+// Every expression keeps the operand order of the reference's optimized kernels
+// (include/backend/openmp/omp_kernels.hpp) so that, compiled with -ffp-contract=off, the
+// results are bitwise equal to the CPU solvers.  Cited per function.
+//
+// The face routine is direction-agnostic: "n" is the normal momentum (hu on x faces, hv on
+// y faces) and "t" the tangential one.  hll_x (omp_kernels.hpp:50-86) and hll_y (:88-124)
+// are exact mirror images under that renaming, so one routine serves both.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace swe {
+
+constexpr double kEps = 1.0e-8;    // include/constants.hpp:5 (safe_velocity threshold)
+constexpr double kFloor = 1.0e-8;  // omp_kernels.hpp:310 (enforce_positivity default floor)
+
+// std::min / std::max, including their NaN and signed-zero behaviour.
+__device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
+__device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
+
+// omp_kernels.hpp:37-39
+__device__ __forceinline__ double positive_height(double h) { return h > 0.0 ? h : 0.0; }
+
+// omp_kernels.hpp:41-43
+__device__ __forceinline__ double safe_velocity(double q, double h) {
+    return (h > kEps) ? (q / h) : 0.0;
+}
+
+// omp_kernels.hpp:45-48.  (a > 0 ? 1 : -1) * m is m or -m exactly, so it is written as a
+// select; the a*b <= 0 test is kept as a product because it can underflow to 0.
+__device__ __forceinline__ double minmod(double a, double b) {
+    const double m = smin(fabs(a), fabs(b));
+    const double r = (a > 0.0) ? m : -m;
+    return (a * b <= 0.0) ? 0.0 : r;
+}
+
+// Result of one face: mass flux, tangential-momentum flux, and the normal-momentum flux as
+// seen from the minus (left/bottom) and plus (right/top) cell, which differ by the
+// hydrostatic pressure correction (omp_kernels.hpp:179-189, :247-257).
+struct FaceFlux {
+    double fh, ft, fn_m, fn_p;
+};
+
+// HLL on the hydrostatically reconstructed states (omp_kernels.hpp:50-86 / :88-124).
+__device__ __forceinline__ void hll(double half_g, double g, double hL, double nL, double tL,
+                                    double hR, double nR, double tR, double &fh, double &fn,
+                                    double &ft) {
+    const double unL = safe_velocity(nL, hL);
+    const double utL = safe_velocity(tL, hL);
+    const double unR = safe_velocity(nR, hR);
+    const double utR = safe_velocity(tR, hR);
+
+    const double cL = sqrt(g * hL);
+    const double cR = sqrt(g * hR);
+
+    const double sL = smin(unL - cL, unR - cR);
+    const double sR = smax(unL + cL, unR + cR);
+
+    const double FL_h = nL;
+    const double FL_n = nL * unL + half_g * hL * hL;
+    const double FL_t = nL * utL;
+    const double FR_h = nR;
+    const double FR_n = nR * unR + half_g * hR * hR;
+    const double FR_t = nR * utR;
+
+    if (sL >= 0.0) {
+        fh = FL_h;
+        fn = FL_n;
+        ft = FL_t;
+    } else if (sR <= 0.0) {
+        fh = FR_h;
+        fn = FR_n;
+        ft = FR_t;
+    } else {
+        const double inv = 1.0 / (sR - sL);
+        fh = (sR * FL_h - sL * FR_h + sL * sR * (hR - hL)) * inv;
+        fn = (sR * FL_n - sL * FR_n + sL * sR * (nR - nL)) * inv;
+        ft = (sR * FL_t - sL * FR_t + sL * sR * (tR - tL)) * inv;
+    }
+}
+
+// One face from the MUSCL-reconstructed minus/plus values of (eta, n, t) and the cell
+// bottoms on either side (omp_kernels.hpp:154-189 for x, :222-257 for y).
+__device__ __forceinline__ FaceFlux face_flux(double half_g, double g, double eta_m, double n_m,
+                                              double t_m, double b_m, double eta_p, double n_p,
+                                              double t_p, double b_p) {
+    const double b_star = smax(b_m, b_p);
+
+    const double h_m = positive_height(eta_m - b_m);
+    const double h_p = positive_height(eta_p - b_p);
+
+    const double un_m = safe_velocity(n_m, h_m);
+    const double ut_m = safe_velocity(t_m, h_m);
+    const double un_p = safe_velocity(n_p, h_p);
+    const double ut_p = safe_velocity(t_p, h_p);
+
+    const double hs_m = positive_height(eta_m - b_star);
+    const double hs_p = positive_height(eta_p - b_star);
+
+    double fh, fn, ft;
+    hll(half_g, g, hs_m, hs_m * un_m, hs_m * ut_m, hs_p, hs_p * un_p, hs_p * ut_p, fh, fn, ft);
+
+    const double corr_m = half_g * (h_m * h_m - hs_m * hs_m);
+    const double corr_p = half_g * (h_p * h_p - hs_p * hs_p);
+
+    FaceFlux f;
+    f.fh = fh;
+    f.ft = ft;
+    f.fn_m = fn + corr_m;
+    f.fn_p = fn + corr_p;
+    return f;
+}
+
+// Divergence of one component (omp_kernels.hpp:281-283):
+//   -inv_dx*(fxm[k] - fxp[k-s]) - inv_dy*(fym[k] - fyp[k-1])
+__device__ __forceinline__ double divergence(double inv_dx, double inv_dy, double fxm,
+                                             double fxp_left, double fym, double fyp_below) {
+    return -inv_dx * (fxm - fxp_left) - inv_dy * (fym - fyp_below);
+}
+
+// SSP-RK3 stage combinations (omp_solver.hpp:264-266, :290-292, :316-320).  `in` is the
+// stage input at the cell (U, U1 or U2), `u0` is U^n.
+template <int STAGE>
+__device__ __forceinline__ double rk_combine(double u0, double in, double dt, double rhs) {
+    if constexpr (STAGE == 1) {
+        return in + dt * rhs;
+    } else if constexpr (STAGE == 2) {
+        return 0.75 * u0 + 0.25 * (in + dt * rhs);
+    } else {
+        return (1.0 / 3.0) * u0 + (2.0 / 3.0) * (in + dt * rhs);
+    }
+}
+
+// Per-cell wave speed of compute_stable_dt (omp_kernels.hpp:297-302): max(|u|+c, |v|+c).
+__device__ __forceinline__ double cell_speed(double g, double h, double hu, double hv,
+                                             double running) {
+    const double u = safe_velocity(hu, h);
+    const double v = safe_velocity(hv, h);
+    const double c = sqrt(g * smax(0.0, h));
+    running = smax(running, fabs(u) + c);
+    running = smax(running, fabs(v) + c);
+    return running;
+}
+
+// compute_stable_dt's final expression (omp_kernels.hpp:305-306).
+__device__ __forceinline__ double dt_from_speed(double max_speed, double cfl, double dx,
+                                                double dy) {
+    if (max_speed <= kEps) return 1.0e100;
+    return cfl * smin(dx, dy) / max_speed;
+}
+
+}  // namespace swe

@@ -1,0 +1,66 @@
+// swe_internal.h -- device-side data layout and kernel launchers shared by the C-ABI.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <cstdint>
+
+namespace swe {
+
+// One grid in the reference layout (omp_kernels.hpp:22-35): row-major, k = i*stride + j,
+// i = x index (slow), j = y index (contiguous), stride = ny + 2*ng.
+struct GridDev {
+    int nx, ny, ng, stride;
+    double dx, dy, inv_dx, inv_dy, g, half_g;
+};
+
+// Arguments of one fused RK stage launch.
+//   in_*  : stage input U_s (read through the radius-2 cross stencil)
+//   u0_*  : U^n (read pointwise; stages 2 and 3)
+//   out_* : stage output (stage 3 may alias u0_*; in_* never aliases out_*)
+struct StageArgs {
+    const double *in_h, *in_hu, *in_hv;
+    const double *u0_h, *u0_hu, *u0_hv;
+    const double *B;
+    double *out_h, *out_hu, *out_hv;
+    const double *dt;   // device scalar
+    double *partial;    // per-block max wave speed (SPEED launches), may be null otherwise
+    int *flag;          // non-finite flag (bit 0)
+    int tile_rows;      // interior rows per wave
+    int nstrips;        // 60-column strips across y
+};
+
+constexpr int kWave = 64;
+constexpr int kStripCols = kWave - 4;  // output columns per wave (2-cell halo each side)
+constexpr int kBlockWaves = 4;
+
+inline int strips_for(int ny) { return (ny + kStripCols - 1) / kStripCols; }
+
+// stage: 0 = right-hand side only (out = rhs), 1..3 = SSP-RK3 stage with floor.
+// bc: 0 = reflecting walls (reference), 1 = transmissive (zero-gradient).
+// speed: also write per-block max wave speed of the output (stage 3 only).
+hipError_t launch_stage(int stage, int bc, bool speed, const GridDev &G, const StageArgs &A,
+                        hipStream_t stream, dim3 *grid_out);
+dim3 stage_grid(const GridDev &G, int tile_rows);
+
+// Max wave speed over the interior of (h, hu, hv) into partial[0..blocks), blocks <= 1024.
+hipError_t launch_speed_reduce(const GridDev &G, const double *h, const double *hu,
+                               const double *hv, double *partial, int blocks,
+                               hipStream_t stream);
+
+// Reduce partial[0..n) to the max speed, then dt = compute_stable_dt expression.
+//   scal[0] = dt, scal[1] = max speed.  If hist != null and *counter < hist_cap,
+//   hist[*counter] = dt; *counter += 1.
+hipError_t launch_finalize_dt(const double *partial, int n, double cfl, double dx, double dy,
+                              double *scal, double *hist, long long *counter, long long hist_cap,
+                              hipStream_t stream);
+
+// Ghost fill of (h, hu, hv) in place (hu == null: scalar fill of h only), reproducing the
+// two-pass order of omp_solver.hpp:340-432 including the corner cells.
+hipError_t launch_ghost_fill(int bc, const GridDev &G, double *h, double *hu, double *hv,
+                             hipStream_t stream);
+
+hipError_t launch_set_scalar(double *dst, double v, hipStream_t stream);
+hipError_t launch_fill_int(int *dst, int v, int n, hipStream_t stream);
+hipError_t launch_fill_double(double *dst, double v, long n, hipStream_t stream);
+
+}  // namespace swe

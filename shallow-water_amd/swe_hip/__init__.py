@@ -1,0 +1,213 @@
+"""Python binding of libswe_hip.so (the MI355X shallow-water backend) over its C ABI.
+
+The C ABI is declared in include/swe_hip.h.  This module only marshals numpy arrays
+(host) or torch tensors (device pointers) through ctypes; all compute runs in the HIP
+kernels.  There is no CPU fallback: if the library is missing or no GPU is visible the
+calls raise.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libswe_hip.so")
+
+SWE_OK = 0
+SWE_ERR_INVALID = 1
+SWE_ERR_HIP = 2
+SWE_ERR_NONFINITE = 3
+SWE_ERR_CFL = 4
+SWE_ERR_OOM = 5
+SWE_ERR_NO_DEVICE = 6
+
+BC_REFLECTING = 0
+BC_TRANSMISSIVE = 1
+
+G_REF = 9.80665  # reference include/constants.hpp:4
+
+_dp = C.POINTER(C.c_double)
+
+
+class GridDesc(C.Structure):
+    _fields_ = [
+        ("nx", C.c_int32), ("ny", C.c_int32), ("ng", C.c_int32),
+        ("dx", C.c_double), ("dy", C.c_double), ("g", C.c_double),
+        ("bc_kind", C.c_int32),
+    ]
+
+
+class SweError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"[{code}] {msg}")
+        self.code = code
+
+
+# every symbol include/swe_hip.h declares, with (restype, argtypes)
+_SIGS = {
+    "swe_hip_abi_version": (C.c_int, []),
+    "swe_hip_error_string": (C.c_char_p, [C.c_int]),
+    "swe_hip_last_error": (C.c_char_p, [C.c_void_p]),
+    "swe_hip_device_count": (C.c_int, []),
+    "swe_hip_create": (C.c_int, [C.POINTER(GridDesc), C.c_int, C.POINTER(C.c_void_p)]),
+    "swe_hip_destroy": (None, [C.c_void_p]),
+    "swe_hip_upload": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp]),
+    "swe_hip_download": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "swe_hip_upload_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p,
+                                        C.c_void_p]),
+    "swe_hip_download_device": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "swe_hip_stable_dt": (C.c_int, [C.c_void_p, C.c_double, _dp, _dp]),
+    "swe_hip_step": (C.c_int, [C.c_void_p, C.c_double, C.c_int64]),
+    "swe_hip_step_adaptive": (C.c_int, [C.c_void_p, C.c_double, C.c_int64, _dp]),
+    "swe_hip_rhs": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
+    "swe_hip_set_tile_rows": (C.c_int, [C.c_void_p, C.c_int]),
+    "swe_hip_get_tile_rows": (C.c_int, [C.c_void_p]),
+    "swe_hip_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "swe_hip_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "swe_hip_time_steps": (C.c_int, [C.c_void_p, C.c_double, C.c_int64, _dp, _dp]),
+    "swe_hip_sync": (C.c_int, [C.c_void_p]),
+}
+
+_lib = None
+
+
+def lib():
+    """Load libswe_hip.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m swe_hip.build` "
+                               "(or __graft_entry__.build())")
+        L = C.CDLL(LIB_PATH)
+        for name, (res, args) in _SIGS.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def device_count() -> int:
+    return lib().swe_hip_device_count()
+
+
+def _p(a: np.ndarray):
+    if a.dtype != np.float64 or not a.flags.c_contiguous:
+        raise TypeError("arrays must be C-contiguous float64")
+    return a.ctypes.data_as(_dp)
+
+
+class HipSolver:
+    """One grid on one GPU.  Mirrors the operations of the reference's backend classes
+    (FastHLLMUSCLBathyOpenMPSolver / FastHLLMUSCLBathyCUDASolver) over raw arrays."""
+
+    def __init__(self, nx, ny, dx, dy, g=G_REF, bc=BC_REFLECTING, ng=2, device=0):
+        self.nx, self.ny, self.ng = int(nx), int(ny), int(ng)
+        self.shape = (self.nx + 2 * self.ng, self.ny + 2 * self.ng)
+        self.desc = GridDesc(self.nx, self.ny, self.ng, float(dx), float(dy), float(g), int(bc))
+        self._ctx = C.c_void_p()
+        L = lib()
+        self._check(L.swe_hip_create(C.byref(self.desc), int(device), C.byref(self._ctx)))
+
+    @classmethod
+    def from_case(cls, case, device=0):
+        return cls(case["nx"], case["ny"], case["dx"], case["dy"], case["g"], case["bc"],
+                   case["ng"], device)
+
+    def _check(self, rc):
+        if rc != SWE_OK:
+            L = lib()
+            detail = ""
+            if self._ctx:
+                detail = (L.swe_hip_last_error(self._ctx) or b"").decode()
+            raise SweError(rc, L.swe_hip_error_string(rc).decode() + (f": {detail}" if detail else ""))
+
+    def close(self):
+        if self._ctx:
+            lib().swe_hip_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def _arr(self, a):
+        a = np.ascontiguousarray(a, dtype=np.float64)
+        if a.shape != self.shape:
+            raise ValueError(f"expected shape {self.shape}, got {a.shape}")
+        return a
+
+    def upload(self, h, hu, hv, B):
+        h, hu, hv, B = (self._arr(a) for a in (h, hu, hv, B))
+        self._check(lib().swe_hip_upload(self._ctx, _p(h), _p(hu), _p(hv), _p(B)))
+
+    def download(self):
+        out = [np.empty(self.shape) for _ in range(3)]
+        self._check(lib().swe_hip_download(self._ctx, *(_p(o) for o in out)))
+        return out
+
+    def upload_device(self, h, hu, hv, B):
+        """torch float64 tensors on this context's device, shape (nx+2ng, ny+2ng)."""
+        for t in (h, hu, hv, B):
+            if tuple(t.shape) != self.shape or not t.is_contiguous():
+                raise ValueError("device tensors must be contiguous with the grid shape")
+        self._check(lib().swe_hip_upload_device(self._ctx, h.data_ptr(), hu.data_ptr(),
+                                                hv.data_ptr(), B.data_ptr()))
+
+    def download_device(self, h, hu, hv):
+        for t in (h, hu, hv):
+            if tuple(t.shape) != self.shape or not t.is_contiguous():
+                raise ValueError("device tensors must be contiguous with the grid shape")
+        self._check(lib().swe_hip_download_device(self._ctx, h.data_ptr(), hu.data_ptr(),
+                                                  hv.data_ptr()))
+
+    def stable_dt(self, cfl):
+        dt, ms = C.c_double(), C.c_double()
+        self._check(lib().swe_hip_stable_dt(self._ctx, float(cfl), C.byref(dt), C.byref(ms)))
+        return dt.value, ms.value
+
+    def step(self, dt, nsteps=1):
+        self._check(lib().swe_hip_step(self._ctx, float(dt), int(nsteps)))
+
+    def step_adaptive(self, cfl, nsteps=1):
+        hist = np.zeros(max(int(nsteps), 1))
+        self._check(lib().swe_hip_step_adaptive(self._ctx, float(cfl), int(nsteps), _p(hist)))
+        return hist[:nsteps]
+
+    def rhs(self):
+        out = [np.empty(self.shape) for _ in range(3)]
+        self._check(lib().swe_hip_rhs(self._ctx, *(_p(o) for o in out)))
+        return out
+
+    @property
+    def tile_rows(self):
+        return lib().swe_hip_get_tile_rows(self._ctx)
+
+    @tile_rows.setter
+    def tile_rows(self, v):
+        self._check(lib().swe_hip_set_tile_rows(self._ctx, int(v)))
+
+    def set_graph(self, enabled: bool):
+        self._check(lib().swe_hip_set_graph(self._ctx, int(bool(enabled))))
+
+    def set_stream(self, stream_handle: int | None):
+        self._check(lib().swe_hip_set_stream(self._ctx, stream_handle or None))
+
+    def time_steps(self, dt, nsteps):
+        st = (C.c_double * 3)()
+        tot = C.c_double()
+        self._check(lib().swe_hip_time_steps(self._ctx, float(dt), int(nsteps), st, C.byref(tot)))
+        return list(st), tot.value
+
+    def sync(self):
+        self._check(lib().swe_hip_sync(self._ctx))

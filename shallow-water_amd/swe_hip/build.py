@@ -1,0 +1,59 @@
+"""Build libswe_hip.so (hipcc, gfx950) in-tree.
+
+    python -m swe_hip.build            # from shallow-water_amd/
+
+The library is written next to this file so it travels with the source tree.
+Flags:
+  -ffp-contract=off   no a*b+c -> fma contraction: keeps the reference's rounding order
+                      (the CPU solvers are built without FMA), so results are bitwise equal.
+  -Xclang -target-feature -Xclang -packed-fp32-ops
+                      no packed-FP32 instructions (none are needed by this FP64 code).
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+PKG_ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(PKG_ROOT, "csrc")
+REPO = os.path.dirname(PKG_ROOT)
+LIB = os.path.join(HERE, "libswe_hip.so")
+
+SOURCES = ["swe_stage.hip", "swe_api.hip"]
+ARCH = os.environ.get("SWE_HIP_ARCH", "gfx950")
+
+HIPCC_FLAGS = [
+    f"--offload-arch={ARCH}", "-O3", "-std=c++17", "-fPIC", "-shared",
+    "-ffp-contract=off",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function",
+]
+
+
+def hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found")
+
+
+def build(verbose: bool = False, extra=None) -> str:
+    srcs = [os.path.join(CSRC, s) for s in SOURCES]
+    tmp = LIB + ".tmp"
+    cmd = [hipcc(), *HIPCC_FLAGS, *(extra or []), "-I", os.path.join(REPO, "include"), "-o", tmp,
+           *srcs]
+    if verbose:
+        print(" ".join(cmd), flush=True)
+    res = subprocess.run(cmd, capture_output=True, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout + res.stderr)
+        raise RuntimeError(f"hipcc failed ({res.returncode})")
+    os.replace(tmp, LIB)
+    return LIB
+
+
+if __name__ == "__main__":
+    print(build(verbose=True))

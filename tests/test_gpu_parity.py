@@ -1,0 +1,268 @@
+"""GPU parity: the HIP fused-stage path against the oracle and the reference's golden vectors.
+
+Bar: bitwise equality (every float64 bit, ghosts included) -- the HIP kernels keep the
+reference's operation order and are compiled without FP contraction.  Runs on an MI355X
+through the C ABI (swe_hip -> libswe_hip.so)."""
+import hashlib
+import json
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+from swe_hip import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint64)
+
+
+def assert_bitwise(got, want, what=""):
+    got, want = np.asarray(got), np.asarray(want)
+    assert got.shape == want.shape, what
+    neq = bits(got) != bits(want)
+    if neq.any():
+        idx = np.argwhere(neq)
+        d = np.abs(got - want)
+        pytest.fail(f"{what}: {neq.sum()} of {neq.size} values differ, first at {tuple(idx[0])}: "
+                    f"got {got[tuple(idx[0])]!r} want {want[tuple(idx[0])]!r}, max |diff| "
+                    f"{np.nanmax(d):.3e}")
+
+
+def oracle_grid(c, bc=None):
+    return oracle.grid(c["nx"], c["ny"], c["ng"], c["dx"], c["dy"], g=c["g"],
+                       bc=c["bc"] if bc is None else bc)
+
+
+def gpu_solver(hip, c):
+    s = hip.HipSolver.from_case(c)
+    s.upload(c["h"], c["hu"], c["hv"], c["B"])
+    return s
+
+
+def oracle_state(c):
+    gr = oracle_grid(c)
+    B = c["B"].copy()
+    oracle.apply_bc(gr, B)
+    h, hu, hv = (c[k].copy() for k in ("h", "hu", "hv"))
+    return gr, h, hu, hv, B
+
+
+RHS_CASES = [
+    ("random_17x13", lambda: W.random_state(17, 13, seed=11)),
+    ("wetdry_32x32", lambda: W.random_state(32, 32, seed=12, wet_dry=True)),
+    ("random_61x130", lambda: W.random_state(61, 130, seed=14)),
+    ("random_130x61_wd", lambda: W.random_state(130, 61, seed=15, wet_dry=True)),
+    ("random_2x2", lambda: W.random_state(2, 2, seed=16)),
+    ("random_5x121_transm", lambda: W.random_state(5, 121, seed=17, bc=1)),
+    ("lake_64_transm", lambda: W.lake_wet_dry(64, seed=2)),
+]
+
+
+@pytest.mark.parametrize("name,make", RHS_CASES, ids=[c[0] for c in RHS_CASES])
+def test_rhs_bitwise(hip, name, make):
+    c = make()
+    gr, h, hu, hv, B = oracle_state(c)
+    want = oracle.rhs(gr, h, hu, hv, B)
+    with gpu_solver(hip, c) as s:
+        got = s.rhs()
+    for comp, g_, w_ in zip(("h", "hu", "hv"), got, want):
+        assert_bitwise(g_, w_, f"{name} rhs_{comp}")
+
+
+@pytest.mark.parametrize("fixture", ["rhs_random_17x13", "rhs_wetdry_32x32", "rhs_lake_24"])
+def test_rhs_golden(hip, fixture):
+    """rhs of the reference's compiled kernels (tests/golden), straight against the GPU."""
+    m = json.load(open(os.path.join(GOLDEN, "manifest.json")))[fixture]
+    z = np.load(os.path.join(GOLDEN, fixture + ".npz"))
+    with hip.HipSolver(m["nx"], m["ny"], m["dx"], m["dy"], hip.G_REF, 0, m["ng"]) as s:
+        s.upload(z["h"], z["hu"], z["hv"], z["B"])
+        got = s.rhs()
+    for comp, g_ in zip(("h", "hu", "hv"), got):
+        assert_bitwise(g_, z["rhs_" + comp], f"{fixture} rhs_{comp}")
+
+
+@pytest.mark.parametrize("fixture", ["steps_gauss_64_10", "steps_gauss_64_100", "steps_lake_48_10",
+                                     "steps_random_20x24_1", "steps_wetdry_48_3"])
+def test_steps_golden(hip, fixture):
+    m = json.load(open(os.path.join(GOLDEN, "manifest.json")))[fixture]
+    z = np.load(os.path.join(GOLDEN, fixture + ".npz"))
+    assert m["rc"] == 0
+    with hip.HipSolver(m["nx"], m["ny"], m["dx"], m["dy"], hip.G_REF, 0, m["ng"]) as s:
+        s.upload(z["h"], z["hu"], z["hv"], z["B"])
+        dt, _ = s.stable_dt(m["cfl"])
+        assert bits(np.array([dt]))[0] == bits(z["dt_stable"])[0]
+        s.step(float(z["dt"][0]), m["steps"])
+        got = s.download()
+    for comp, g_ in zip(("h", "hu", "hv"), got):
+        assert_bitwise(g_, z["out_" + comp], f"{fixture} {comp}")
+
+
+def test_adaptive_golden(hip):
+    fixture = "adaptive_gauss_48_20"
+    m = json.load(open(os.path.join(GOLDEN, "manifest.json")))[fixture]
+    z = np.load(os.path.join(GOLDEN, fixture + ".npz"))
+    with hip.HipSolver(m["nx"], m["ny"], m["dx"], m["dy"], hip.G_REF, 0, m["ng"]) as s:
+        s.upload(z["h"], z["hu"], z["hv"], z["B"])
+        hist = s.step_adaptive(m["cfl"], m["steps"])
+        got = s.download()
+    assert_bitwise(hist, z["dt_hist"], "dt history")
+    for comp, g_ in zip(("h", "hu", "hv"), got):
+        assert_bitwise(g_, z["out_" + comp], f"{fixture} {comp}")
+
+
+def test_gauss_256_1000_reference_hash(hip):
+    """C1 geometry at g = 9.80665, 1000 steps: sha256 of the reference's final fields."""
+    m = json.load(open(os.path.join(GOLDEN, "manifest.json")))["steps_gauss_256_1000"]
+    c = W.gauss_pulse(256, g=hip.G_REF)
+    gr, h, hu, hv, B = oracle_state(c)
+    oracle.apply_bc(gr, h, hu, hv)
+    got_in = {"h": h, "hu": hu, "hv": hv, "B": B}
+    for k, v in got_in.items():
+        if hashlib.sha256(v.tobytes()).hexdigest() != m["sha256_in"][k]:
+            pytest.skip(f"generated input {k} differs from the fixture's (libm exp differs)")
+    with gpu_solver(hip, c) as s:
+        dt, _ = s.stable_dt(0.4)
+        assert dt == m["dt_stable"]
+        s.step(dt, 1000)
+        out = s.download()
+    for comp, a in zip(("h", "hu", "hv"), out):
+        assert hashlib.sha256(a.tobytes()).hexdigest() == m["sha256_out"][comp], comp
+
+
+def test_gauss_256_g981_vs_oracle(hip):
+    """Config 1 as stated (g = 9.81, CFL 0.4 -> fixed dt, 1000 steps), bitwise vs oracle."""
+    c = W.gauss_pulse(256, g=9.81)
+    gr, h, hu, hv, B = oracle_state(c)
+    dt_o, _ = oracle.stable_dt(gr, h, hu, hv, 0.4)
+    with gpu_solver(hip, c) as s:
+        dt, _ = s.stable_dt(0.4)
+        assert dt == dt_o
+        s.step(dt, 1000)
+        got = s.download()
+    rc, _, _ = oracle.run(gr, h, hu, hv, B, 1000, dt=dt_o)
+    assert rc == 0
+    for comp, g_, w_ in zip(("h", "hu", "hv"), got, (h, hu, hv)):
+        assert_bitwise(g_, w_, f"gauss_256 g=9.81 {comp}")
+
+
+@pytest.mark.parametrize("bc", [0, 1])
+def test_adaptive_vs_oracle(hip, bc):
+    c = W.ensemble_member(3, 150)
+    c["bc"] = bc
+    gr, h, hu, hv, B = oracle_state(c)
+    with gpu_solver(hip, c) as s:
+        hist = s.step_adaptive(0.4, 40)
+        got = s.download()
+    rc, _, want_hist = oracle.run(gr, h, hu, hv, B, 40, cfl=0.4)
+    assert rc == 0
+    assert_bitwise(hist, want_hist, "dt history")
+    for comp, g_, w_ in zip(("h", "hu", "hv"), got, (h, hu, hv)):
+        assert_bitwise(g_, w_, f"adaptive bc={bc} {comp}")
+
+
+@pytest.mark.parametrize("tile_rows", [1, 3, 16, 64, 1000])
+def test_tile_rows_invariant(hip, tile_rows):
+    c = W.lake_wet_dry(97, seed=6, eta0=2.0)
+    gr, h, hu, hv, B = oracle_state(c)
+    dt, _ = oracle.stable_dt(gr, h, hu, hv, 0.4)
+    with gpu_solver(hip, c) as s:
+        s.tile_rows = tile_rows
+        s.step(dt, 7)
+        got = s.download()
+    oracle.run(gr, h, hu, hv, B, 7, dt=dt)
+    for comp, g_, w_ in zip(("h", "hu", "hv"), got, (h, hu, hv)):
+        assert_bitwise(g_, w_, f"tile_rows={tile_rows} {comp}")
+
+
+def test_graph_and_direct_launch_agree(hip):
+    c = W.ensemble_member(7, 90)
+    outs = []
+    for graph in (True, False):
+        with gpu_solver(hip, c) as s:
+            s.set_graph(graph)
+            dt, _ = s.stable_dt(0.4)
+            s.step(dt, 5)
+            s.step(dt, 6)
+            outs.append(s.download())
+    for a, b in zip(*outs):
+        assert_bitwise(a, b, "graph vs direct")
+
+
+@pytest.mark.parametrize("g", [9.80665, 9.81])
+def test_stable_dt_bitwise(hip, g):
+    for seed in range(3):
+        c = W.random_state(40 + seed, 70, seed=20 + seed, wet_dry=bool(seed & 1), g=g)
+        gr, h, hu, hv, B = oracle_state(c)
+        want = oracle.stable_dt(gr, h, hu, hv, 0.45)
+        with gpu_solver(hip, c) as s:
+            got = s.stable_dt(0.45)
+        assert got == want
+
+
+def test_nonfinite_is_reported(hip):
+    c = W.gauss_pulse(64, g=hip.G_REF)
+    c["hu"][30, 30] = np.nan
+    with gpu_solver(hip, c) as s:
+        with pytest.raises(hip.SweError) as e:
+            s.step(1e-4, 2)
+        assert e.value.code == hip.SWE_ERR_NONFINITE
+
+
+def test_wetdry_divergence_matches_reference(hip):
+    """Wet/dry lake (eta 0.5 over B up to 1.6): the reference scheme diverges after a few
+    steps; the GPU reproduces the state bitwise up to then and reports the same failure."""
+    m = json.load(open(os.path.join(GOLDEN, "manifest.json")))
+    z = np.load(os.path.join(GOLDEN, "steps_wetdry_48_3.npz"))
+    assert m["steps_wetdry_48_10"]["rc"] == 1
+    dx = m["steps_wetdry_48_3"]["dx"]
+    with hip.HipSolver(48, 48, dx, dx, hip.G_REF, 0, 2) as s:
+        s.upload(z["h"], z["hu"], z["hv"], z["B"])
+        s.step(float(z["dt"][0]), 3)
+        got = s.download()
+        for comp, g_ in zip(("h", "hu", "hv"), got):
+            assert_bitwise(g_, z["out_" + comp], f"wetdry 3 steps {comp}")
+        with pytest.raises(hip.SweError) as e:
+            s.step(float(z["dt"][0]), 7)
+        assert e.value.code == hip.SWE_ERR_NONFINITE
+
+
+def test_lake_at_rest_wet_roundoff(hip):
+    """Fully wet lake at rest over 32 Gaussian bumps, transmissive walls: stays at rest to
+    round-off and matches the oracle bitwise."""
+    c = W.lake_wet_dry(256, seed=2, bump=False, eta0=2.0)
+    gr, h, hu, hv, B = oracle_state(c)
+    dt, _ = oracle.stable_dt(gr, h, hu, hv, 0.4)
+    with gpu_solver(hip, c) as s:
+        s.step(dt, 200)
+        got = s.download()
+    oracle.run(gr, h, hu, hv, B, 200, dt=dt)
+    for comp, g_, w_ in zip(("h", "hu", "hv"), got, (h, hu, hv)):
+        assert_bitwise(g_, w_, f"lake {comp}")
+    I = (slice(2, 258), slice(2, 258))
+    eta = got[0][I] + B[I]
+    assert np.abs(eta - 2.0).max() < 1e-12
+    assert np.abs(got[1][I]).max() < 1e-11 and np.abs(got[2][I]).max() < 1e-11
+
+
+@pytest.mark.slow
+def test_dam_break_8192_step_subset(hip):
+    """Config 3 at its real size (8192^2), 2 steps, bitwise vs the oracle."""
+    c = W.dam_break(8192)
+    gr, h, hu, hv, B = oracle_state(c)
+    dt, _ = oracle.stable_dt(gr, h, hu, hv, 0.45)
+    with gpu_solver(hip, c) as s:
+        dtg, _ = s.stable_dt(0.45)
+        assert dtg == dt
+        s.step(dt, 2)
+        got = s.download()
+    del c
+    rc, _, _ = oracle.run(gr, h, hu, hv, B, 2, dt=dt)
+    assert rc == 0
+    for comp, g_, w_ in zip(("h", "hu", "hv"), got, (h, hu, hv)):
+        assert_bitwise(g_, w_, f"dam_break 8192 {comp}")
