@@ -41,6 +41,12 @@ typedef enum {
 } swe_status;
 
 typedef enum {
+    SWE_PRECISION_EXACT = 0, /* the reference's arithmetic, bitwise equal to omp_solver */
+    SWE_PRECISION_FAST = 1   /* reciprocal-based velocities; differs from the reference by a
+                                few ulps per step (bounded by the tests after N steps) */
+} swe_precision;
+
+typedef enum {
     SWE_BC_REFLECTING = 0,  /* reference walls (omp_solver.hpp:340-432) */
     SWE_BC_TRANSMISSIVE = 1 /* zero-gradient copy; extension, no reference counterpart */
 } swe_bc_kind;
@@ -113,6 +119,11 @@ int swe_hip_rhs(swe_hip_ctx *ctx, double *rhs_h, double *rhs_hu, double *rhs_hv)
  * Default picked by swe_hip_create from the grid size. */
 int swe_hip_set_tile_rows(swe_hip_ctx *ctx, int tile_rows);
 int swe_hip_get_tile_rows(const swe_hip_ctx *ctx);
+
+/* Arithmetic of the stage kernels (swe_precision); default SWE_PRECISION_EXACT.  No
+ * counterpart in the reference: FAST trades bitwise parity for fewer FP64 divisions. */
+int swe_hip_set_precision(swe_hip_ctx *ctx, int mode);
+int swe_hip_get_precision(const swe_hip_ctx *ctx);
 
 /* Use a caller stream (hipStream_t passed as void*; NULL = the context's own stream). */
 int swe_hip_set_stream(swe_hip_ctx *ctx, void *stream);

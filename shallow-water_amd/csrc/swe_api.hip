@@ -39,11 +39,13 @@ struct swe_hip_ctx {
     double *hist = nullptr;
     long long hist_cap = 0;
     int tile_rows = 64;
+    int precision = SWE_PRECISION_EXACT;
     bool use_graph = true;
     // cached graph of one step: key = (mode, tile_rows, stream, hist pointer)
     hipGraphExec_t graph = nullptr;
     int graph_mode = -1;
     int graph_tile = -1;
+    int graph_precision = -1;
     hipStream_t graph_stream = nullptr;
     double graph_cfl = 0.0;
     std::string err;
@@ -67,16 +69,17 @@ int fail(swe_hip_ctx *c, int code, const std::string &msg) {
     } while (0)
 
 int default_tile_rows(const GridDev &G) {
-    // Enough waves to fill 256 CUs several times over, long enough marches to amortise
-    // the 4-row halo of each segment.
+    // 32 rows per wave measured best on 8192^2 (vs 64/128/256: fewer waves idle at the end
+    // of the launch); smaller grids halve it down to 4 to keep ~3 waves per SIMD busy
+    // (256 CUs x 4 SIMDs x 3 waves), trading the 4-row prologue of each segment.
     const long strips = swe::strips_for(G.ny);
-    int tr = 128;
-    while (tr > 16 && strips * ((G.nx + tr - 1) / tr) < 8L * 256 * 4) tr /= 2;
+    int tr = 32;
+    while (tr > 4 && strips * ((G.nx + tr - 1) / tr) < 3L * 256 * 4) tr /= 2;
     return tr;
 }
 
 void drop_graph(swe_hip_ctx *c) {
-    if (c->graph) hipGraphExecDestroy(c->graph);
+    if (c->graph) (void)hipGraphExecDestroy(c->graph);
     c->graph = nullptr;
     c->graph_mode = -1;
 }
@@ -84,7 +87,7 @@ void drop_graph(swe_hip_ctx *c) {
 int ensure_partial(swe_hip_ctx *c, int need) {
     if (need <= c->partial_cap) return SWE_OK;
     drop_graph(c);  // a captured step bakes the partial-max pointer
-    if (c->partial) hipFree(c->partial);
+    if (c->partial) (void)hipFree(c->partial);
     c->partial = nullptr;
     c->partial_cap = 0;
     SWE_CHECK(c, hipMalloc(&c->partial, sizeof(double) * (size_t)need));
@@ -122,7 +125,8 @@ int enqueue_step(swe_hip_ctx *c, int mode, double cfl) {
     for (int stage = 1; stage <= 3; ++stage) {
         StageArgs a = stage_args(c, stage);
         dim3 grid;
-        SWE_CHECK(c, swe::launch_stage(stage, c->bc, speed && stage == 3, c->G, a, c->stream,
+        SWE_CHECK(c, swe::launch_stage(stage, c->bc, speed && stage == 3,
+                                       c->precision == SWE_PRECISION_FAST, c->G, a, c->stream,
                                        &grid));
         if (speed && stage == 3) {
             SWE_CHECK(c, swe::launch_finalize_dt(c->partial, (int)(grid.x * grid.y), cfl, c->G.dx,
@@ -142,7 +146,8 @@ int run_steps(swe_hip_ctx *c, int mode, double cfl, int64_t nsteps) {
         return SWE_OK;
     }
     if (!c->graph || c->graph_mode != mode || c->graph_tile != c->tile_rows ||
-        c->graph_stream != c->stream || (mode == 1 && c->graph_cfl != cfl)) {
+        c->graph_precision != c->precision || c->graph_stream != c->stream ||
+        (mode == 1 && c->graph_cfl != cfl)) {
         drop_graph(c);
         hipGraph_t g = nullptr;
         SWE_CHECK(c, hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
@@ -150,16 +155,17 @@ int run_steps(swe_hip_ctx *c, int mode, double cfl, int64_t nsteps) {
         hipGraph_t captured = nullptr;
         const hipError_t e = hipStreamEndCapture(c->stream, &captured);
         if (rc) {
-            if (captured) hipGraphDestroy(captured);
+            if (captured) (void)hipGraphDestroy(captured);
             return rc;
         }
         SWE_CHECK(c, e);
         g = captured;
         const hipError_t ei = hipGraphInstantiate(&c->graph, g, nullptr, nullptr, 0);
-        hipGraphDestroy(g);
+        (void)hipGraphDestroy(g);
         SWE_CHECK(c, ei);
         c->graph_mode = mode;
         c->graph_tile = c->tile_rows;
+        c->graph_precision = c->precision;
         c->graph_stream = c->stream;
         c->graph_cfl = cfl;
     }
@@ -274,21 +280,21 @@ int swe_hip_create(const swe_grid_desc *d, int device, swe_hip_ctx **out) {
 
 void swe_hip_destroy(swe_hip_ctx *c) {
     if (!c) return;
-    hipSetDevice(c->device);
-    if (c->stream) hipStreamSynchronize(c->stream);
+    (void)hipSetDevice(c->device);
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
     drop_graph(c);
     for (int i = 0; i < 3; ++i) {
-        hipFree(c->U[i]);
-        hipFree(c->U1[i]);
-        hipFree(c->U2[i]);
+        (void)hipFree(c->U[i]);
+        (void)hipFree(c->U1[i]);
+        (void)hipFree(c->U2[i]);
     }
-    hipFree(c->B);
-    hipFree(c->scal);
-    hipFree(c->partial);
-    hipFree(c->flag);
-    hipFree(c->counter);
-    hipFree(c->hist);
-    if (c->own_stream) hipStreamDestroy(c->own_stream);
+    (void)hipFree(c->B);
+    (void)hipFree(c->scal);
+    (void)hipFree(c->partial);
+    (void)hipFree(c->flag);
+    (void)hipFree(c->counter);
+    (void)hipFree(c->hist);
+    if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
 }
 
@@ -365,7 +371,7 @@ int swe_hip_step_adaptive(swe_hip_ctx *c, double cfl, int64_t nsteps, double *dt
     if (rc) return rc;
     if (c->hist_cap < nsteps + 1) {
         drop_graph(c);  // the graph bakes the history pointer
-        if (c->hist) hipFree(c->hist);
+        if (c->hist) (void)hipFree(c->hist);
         c->hist = nullptr;
         c->hist_cap = 0;
         SWE_CHECK(c, hipMalloc(&c->hist, sizeof(double) * (size_t)(nsteps + 1)));
@@ -393,7 +399,8 @@ int swe_hip_rhs(swe_hip_ctx *c, double *rh, double *rhu, double *rhv) {
     for (int i = 0; i < 3; ++i)
         SWE_CHECK(c, swe::launch_fill_double(c->U1[i], 0.0, (long)c->n_total, c->stream));
     StageArgs a = stage_args(c, 0);
-    SWE_CHECK(c, swe::launch_stage(0, c->bc, false, c->G, a, c->stream, nullptr));
+    SWE_CHECK(c, swe::launch_stage(0, c->bc, false, c->precision == SWE_PRECISION_FAST, c->G, a,
+                                   c->stream, nullptr));
     const size_t bytes = c->n_total * sizeof(double);
     SWE_CHECK(c, hipMemcpyAsync(rh, c->U1[0], bytes, hipMemcpyDeviceToHost, c->stream));
     SWE_CHECK(c, hipMemcpyAsync(rhu, c->U1[1], bytes, hipMemcpyDeviceToHost, c->stream));
@@ -409,6 +416,14 @@ int swe_hip_set_tile_rows(swe_hip_ctx *c, int tile_rows) {
 }
 
 int swe_hip_get_tile_rows(const swe_hip_ctx *c) { return c ? c->tile_rows : -1; }
+
+int swe_hip_set_precision(swe_hip_ctx *c, int mode) {
+    if (!c || (mode != SWE_PRECISION_EXACT && mode != SWE_PRECISION_FAST)) return SWE_ERR_INVALID;
+    c->precision = mode;
+    return SWE_OK;
+}
+
+int swe_hip_get_precision(const swe_hip_ctx *c) { return c ? c->precision : -1; }
 
 int swe_hip_set_stream(swe_hip_ctx *c, void *stream) {
     if (!c) return SWE_ERR_INVALID;
@@ -436,7 +451,9 @@ int swe_hip_time_steps(swe_hip_ctx *c, double dt, int64_t nsteps, double *stage_
     for (int64_t s = 0; s < nsteps && rc == SWE_OK; ++s) {
         for (int stage = 1; stage <= 3; ++stage) {
             StageArgs a = stage_args(c, stage);
-            const hipError_t e = swe::launch_stage(stage, c->bc, false, c->G, a, c->stream, nullptr);
+            const hipError_t e = swe::launch_stage(stage, c->bc, false,
+                                                   c->precision == SWE_PRECISION_FAST, c->G, a,
+                                                   c->stream, nullptr);
             if (e != hipSuccess) {
                 rc = fail(c, SWE_ERR_HIP, std::string("launch_stage: ") + hipGetErrorString(e));
                 break;
@@ -460,7 +477,7 @@ int swe_hip_time_steps(swe_hip_ctx *c, double dt, int64_t nsteps, double *stage_
             for (int i = 0; i < 3; ++i) stage_ms[i] = acc[i] / (double)nsteps;
         if (total_ms) *total_ms = tot;
     }
-    for (auto &e : ev) hipEventDestroy(e);
+    for (auto &e : ev) (void)hipEventDestroy(e);
     if (rc) return rc;
     return check_flag(c);
 }

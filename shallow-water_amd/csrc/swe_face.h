@@ -21,19 +21,20 @@ constexpr double kFloor = 1.0e-8;  // omp_kernels.hpp:310 (enforce_positivity de
 __device__ __forceinline__ double smin(double a, double b) { return (b < a) ? b : a; }
 __device__ __forceinline__ double smax(double a, double b) { return (a < b) ? b : a; }
 
-// omp_kernels.hpp:37-39
-__device__ __forceinline__ double positive_height(double h) { return h > 0.0 ? h : 0.0; }
+// omp_kernels.hpp:37-39.  v_max_f64 orders -0 below +0 and drops a NaN operand, so
+// fmax(h, 0) is +0 for h = -0, 0 and NaN exactly like `h > 0 ? h : 0`.
+__device__ __forceinline__ double positive_height(double h) { return fmax(h, 0.0); }
 
 // omp_kernels.hpp:41-43
 __device__ __forceinline__ double safe_velocity(double q, double h) {
     return (h > kEps) ? (q / h) : 0.0;
 }
 
-// omp_kernels.hpp:45-48.  (a > 0 ? 1 : -1) * m is m or -m exactly, so it is written as a
-// select; the a*b <= 0 test is kept as a product because it can underflow to 0.
+// omp_kernels.hpp:45-48.  When a*b > 0, a is non-zero, so (a > 0 ? 1 : -1) * m equals
+// copysign(m, a); std::min of two non-negative values equals fmin.  The a*b <= 0 test is
+// kept as a product because it can underflow to 0.
 __device__ __forceinline__ double minmod(double a, double b) {
-    const double m = smin(fabs(a), fabs(b));
-    const double r = (a > 0.0) ? m : -m;
+    const double r = copysign(fmin(fabs(a), fabs(b)), a);
     return (a * b <= 0.0) ? 0.0 : r;
 }
 
@@ -44,20 +45,46 @@ struct FaceFlux {
     double fh, ft, fn_m, fn_p;
 };
 
+// 1/d from the hardware reciprocal estimate refined by two Newton steps (relative error
+// within about one ulp).  Used only by the FAST precision mode.
+__device__ __forceinline__ double recip(double d) {
+    const double r0 = __builtin_amdgcn_rcp(d);
+    const double r1 = fma(r0, fma(-d, r0, 1.0), r0);
+    return fma(r1, fma(-d, r1, 1.0), r1);
+}
+
 // HLL on the hydrostatically reconstructed states (omp_kernels.hpp:50-86 / :88-124).
+//
+// FAST = false: the reference arithmetic, every division IEEE (bitwise parity).
+// FAST = true : the velocities of the reconstructed states are taken from the caller
+//   (vnL.. = the face velocities before the hydrostatic depth change, so hs*u/hs -> u, a
+//   1-ulp difference) and 1/(sR - sL) is a refined reciprocal.  Results then differ from
+//   the reference by a few ulps per step, which the tests bound after N steps.
+template <bool FAST>
 __device__ __forceinline__ void hll(double half_g, double g, double hL, double nL, double tL,
-                                    double hR, double nR, double tR, double &fh, double &fn,
+                                    double hR, double nR, double tR, double vnL, double vtL,
+                                    double vnR, double vtR, double &fh, double &fn,
                                     double &ft) {
-    const double unL = safe_velocity(nL, hL);
-    const double utL = safe_velocity(tL, hL);
-    const double unR = safe_velocity(nR, hR);
-    const double utR = safe_velocity(tR, hR);
+    double unL, utL, unR, utR;
+    if constexpr (FAST) {
+        unL = hL > kEps ? vnL : 0.0;
+        utL = hL > kEps ? vtL : 0.0;
+        unR = hR > kEps ? vnR : 0.0;
+        utR = hR > kEps ? vtR : 0.0;
+    } else {
+        unL = safe_velocity(nL, hL);
+        utL = safe_velocity(tL, hL);
+        unR = safe_velocity(nR, hR);
+        utR = safe_velocity(tR, hR);
+    }
 
     const double cL = sqrt(g * hL);
     const double cR = sqrt(g * hR);
 
-    const double sL = smin(unL - cL, unR - cR);
-    const double sR = smax(unL + cL, unR + cR);
+    // fmin/fmax instead of std::min/max: they differ only in the sign of a zero result or
+    // on NaN, and a zero sL (sR) takes the sL >= 0 (sR <= 0) branch where its value is unused.
+    const double sL = fmin(unL - cL, unR - cR);
+    const double sR = fmax(unL + cL, unR + cR);
 
     const double FL_h = nL;
     const double FL_n = nL * unL + half_g * hL * hL;
@@ -75,7 +102,7 @@ __device__ __forceinline__ void hll(double half_g, double g, double hL, double n
         fn = FR_n;
         ft = FR_t;
     } else {
-        const double inv = 1.0 / (sR - sL);
+        const double inv = FAST ? recip(sR - sL) : 1.0 / (sR - sL);
         fh = (sR * FL_h - sL * FR_h + sL * sR * (hR - hL)) * inv;
         fn = (sR * FL_n - sL * FR_n + sL * sR * (nR - nL)) * inv;
         ft = (sR * FL_t - sL * FR_t + sL * sR * (tR - tL)) * inv;
@@ -84,24 +111,38 @@ __device__ __forceinline__ void hll(double half_g, double g, double hL, double n
 
 // One face from the MUSCL-reconstructed minus/plus values of (eta, n, t) and the cell
 // bottoms on either side (omp_kernels.hpp:154-189 for x, :222-257 for y).
+template <bool FAST>
 __device__ __forceinline__ FaceFlux face_flux(double half_g, double g, double eta_m, double n_m,
                                               double t_m, double b_m, double eta_p, double n_p,
                                               double t_p, double b_p) {
-    const double b_star = smax(b_m, b_p);
+    // fmax: differs from std::max only for (b_m, b_p) = (-0, +0), where eta - b_star then
+    // differs only in the sign of a zero difference, which positive_height maps to +0.
+    const double b_star = fmax(b_m, b_p);
 
     const double h_m = positive_height(eta_m - b_m);
     const double h_p = positive_height(eta_p - b_p);
 
-    const double un_m = safe_velocity(n_m, h_m);
-    const double ut_m = safe_velocity(t_m, h_m);
-    const double un_p = safe_velocity(n_p, h_p);
-    const double ut_p = safe_velocity(t_p, h_p);
+    double un_m, ut_m, un_p, ut_p;
+    if constexpr (FAST) {
+        // one refined reciprocal per depth instead of two divisions
+        const double rm = recip(h_m), rp = recip(h_p);
+        un_m = h_m > kEps ? n_m * rm : 0.0;
+        ut_m = h_m > kEps ? t_m * rm : 0.0;
+        un_p = h_p > kEps ? n_p * rp : 0.0;
+        ut_p = h_p > kEps ? t_p * rp : 0.0;
+    } else {
+        un_m = safe_velocity(n_m, h_m);
+        ut_m = safe_velocity(t_m, h_m);
+        un_p = safe_velocity(n_p, h_p);
+        ut_p = safe_velocity(t_p, h_p);
+    }
 
     const double hs_m = positive_height(eta_m - b_star);
     const double hs_p = positive_height(eta_p - b_star);
 
     double fh, fn, ft;
-    hll(half_g, g, hs_m, hs_m * un_m, hs_m * ut_m, hs_p, hs_p * un_p, hs_p * ut_p, fh, fn, ft);
+    hll<FAST>(half_g, g, hs_m, hs_m * un_m, hs_m * ut_m, hs_p, hs_p * un_p, hs_p * ut_p, un_m,
+              ut_m, un_p, ut_p, fh, fn, ft);
 
     const double corr_m = half_g * (h_m * h_m - hs_m * hs_m);
     const double corr_p = half_g * (h_p * h_p - hs_p * hs_p);
@@ -140,8 +181,9 @@ __device__ __forceinline__ double cell_speed(double g, double h, double hu, doub
     const double u = safe_velocity(hu, h);
     const double v = safe_velocity(hv, h);
     const double c = sqrt(g * smax(0.0, h));
-    running = smax(running, fabs(u) + c);
-    running = smax(running, fabs(v) + c);
+    // all operands are >= +0 here, where fmax equals std::max
+    running = fmax(running, fabs(u) + c);
+    running = fmax(running, fabs(v) + c);
     return running;
 }
 

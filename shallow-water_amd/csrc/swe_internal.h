@@ -38,8 +38,9 @@ inline int strips_for(int ny) { return (ny + kStripCols - 1) / kStripCols; }
 // stage: 0 = right-hand side only (out = rhs), 1..3 = SSP-RK3 stage with floor.
 // bc: 0 = reflecting walls (reference), 1 = transmissive (zero-gradient).
 // speed: also write per-block max wave speed of the output (stage 3 only).
-hipError_t launch_stage(int stage, int bc, bool speed, const GridDev &G, const StageArgs &A,
-                        hipStream_t stream, dim3 *grid_out);
+// fast: reciprocal-based velocities (few-ulp differences from the reference arithmetic).
+hipError_t launch_stage(int stage, int bc, bool speed, bool fast, const GridDev &G,
+                        const StageArgs &A, hipStream_t stream, dim3 *grid_out);
 dim3 stage_grid(const GridDev &G, int tile_rows);
 
 // Max wave speed over the interior of (h, hu, hv) into partial[0..blocks), blocks <= 1024.

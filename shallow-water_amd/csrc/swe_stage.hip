@@ -31,9 +31,20 @@ struct Cell {
     double h, hu, hv, b, eta;
 };
 
-// Lane l receives lane l+1's value / lane l-1's value.  Only lanes 1..62 consume them.
-__device__ __forceinline__ double from_next(double x) { return __shfl_down(x, 1); }
-__device__ __forceinline__ double from_prev(double x) { return __shfl_up(x, 1); }
+// Lane l receives lane l+1's value / lane l-1's value, through the DPP wavefront shifts
+// wave_shl:1 (0x130) and wave_shr:1 (0x138) of the GFX9 ISA: a VALU move, no LDS round
+// trip.  The lane without a source (63 resp. 0) keeps its own value; only lanes 1..62
+// consume the results.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double x) {
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(x), __double2loint(x), CTRL, 0xF,
+                                               0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(x), __double2hiint(x), CTRL, 0xF,
+                                               0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double from_next(double x) { return dpp_move<0x130>(x); }
+__device__ __forceinline__ double from_prev(double x) { return dpp_move<0x138>(x); }
 
 template <int BC>
 __device__ __forceinline__ int mirror(int r, int lo, int hi, bool &ghost) {
@@ -67,8 +78,111 @@ __device__ __forceinline__ Cell load_cell(const StageArgs &A, int row_off, bool 
     return c;
 }
 
-template <int STAGE, int BC, bool SPEED>
-__global__ __launch_bounds__(256) void stage_kernel(const GridDev G, const StageArgs A) {
+struct V3 {
+    double e, u, v;  // (eta, hu, hv) or, for fluxes, (h, hu, hv) components
+};
+
+// State of one wave's march down its strip, and the per-row step.  The step for output row
+// r sees the cells of rows r (a), r+1 (b), r+2 (n) and loads row r+3 into `nn`; it also
+// takes the slope of row r (sa) and d_{r+1/2} = b - a (d2), and produces the slope of row
+// r+1 (sb) and d_{r+3/2} (d3); fxp is the plus-side flux of face r-1/2, fxn receives that
+// of face r+1/2.  The caller rotates the register names, so nothing is copied.
+template <int STAGE, int BC, bool SPEED, bool FAST>
+struct March {
+    const StageArgs &A;
+    int s, ng, nx, rmax, r1, j, jc;
+    bool gcol, owner;
+    double half_g, g, inv_dx, inv_dy, dt;
+    double speed;
+    int bad;
+
+    __device__ __forceinline__ Cell load(int r) const {
+        bool gr;
+        const int rs = mirror<BC>(r, ng, ng + nx, gr);
+        return load_cell<BC>(A, rs * s, gr, jc, gcol);
+    }
+
+    __device__ __forceinline__ V3 load_u0(int r) const {
+        const int k = r * s + jc;
+        return V3{A.u0_h[k], A.u0_hu[k], A.u0_hv[k]};
+    }
+
+    __device__ __forceinline__ void row(int r, const Cell &a, const Cell &b, const Cell &n,
+                                        Cell &nn, const V3 &sa, V3 &sb, const V3 &d2, V3 &d3,
+                                        const V3 &fxp, V3 &fxn, const V3 &u0, V3 &u0n) {
+        // prefetch row r+3 and U^n at row r+1
+        nn = load(r + 3 <= rmax ? r + 3 : rmax);
+        if (STAGE >= 2) u0n = load_u0(r + 1 < r1 ? r + 1 : r);
+
+        // ---- x: slope of row r+1, face r+1/2
+        d3 = V3{n.eta - b.eta, n.hu - b.hu, n.hv - b.hv};
+        sb = V3{0.5 * minmod(d2.e, d3.e), 0.5 * minmod(d2.u, d3.u), 0.5 * minmod(d2.v, d3.v)};
+        const FaceFlux fx = face_flux<FAST>(half_g, g, a.eta + sa.e, a.hu + sa.u, a.hv + sa.v, a.b,
+                                      b.eta - sb.e, b.hu - sb.u, b.hv - sb.v, b.b);
+        fxn = V3{fx.fh, fx.fn_p, fx.ft};
+
+        // ---- y: neighbours along the lanes, face j+1/2 in lane j
+        const double dy_e = from_next(a.eta) - a.eta;
+        const double dy_u = from_next(a.hu) - a.hu;
+        const double dy_v = from_next(a.hv) - a.hv;
+        const double sy_e = 0.5 * minmod(from_prev(dy_e), dy_e);
+        const double sy_u = 0.5 * minmod(from_prev(dy_u), dy_u);
+        const double sy_v = 0.5 * minmod(from_prev(dy_v), dy_v);
+        const double pe = from_next(a.eta - sy_e);
+        const double pu = from_next(a.hu - sy_u);
+        const double pv = from_next(a.hv - sy_v);
+        const double pb = from_next(a.b);
+        // y faces: normal momentum is hv, tangential hu
+        const FaceFlux fy = face_flux<FAST>(half_g, g, a.eta + sy_e, a.hv + sy_v, a.hu + sy_u, a.b, pe,
+                                      pv, pu, pb);
+        const double fyp_h = from_prev(fy.fh);
+        const double fyp_hu = from_prev(fy.ft);
+        const double fyp_hv = from_prev(fy.fn_p);
+
+        // ---- divergence (omp_kernels.hpp:281-283)
+        const double rh = divergence(inv_dx, inv_dy, fx.fh, fxp.e, fy.fh, fyp_h);
+        const double rhu = divergence(inv_dx, inv_dy, fx.fn_m, fxp.u, fy.ft, fyp_hu);
+        const double rhv = divergence(inv_dx, inv_dy, fx.ft, fxp.v, fy.fn_m, fyp_hv);
+
+        if (owner) {
+            const int k = r * s + j;
+            if (STAGE == 0) {
+                A.out_h[k] = rh;
+                A.out_hu[k] = rhu;
+                A.out_hv[k] = rhv;
+            } else {
+                const double s0h = STAGE == 1 ? a.h : u0.e;
+                const double s0u = STAGE == 1 ? a.hu : u0.u;
+                const double s0v = STAGE == 1 ? a.hv : u0.v;
+                double hN = rk_combine<STAGE>(s0h, a.h, dt, rh);
+                double huN = rk_combine<STAGE>(s0u, a.hu, dt, rhu);
+                double hvN = rk_combine<STAGE>(s0v, a.hv, dt, rhv);
+                // enforce_positivity (omp_kernels.hpp:317-325)
+                if (hN < kFloor) {
+                    hN = kFloor;
+                    huN = 0.0;
+                    hvN = 0.0;
+                }
+                if (!isfinite(hN) || !isfinite(huN) || !isfinite(hvN)) bad = 1;
+                if (SPEED) speed = cell_speed(g, hN, huN, hvN, speed);
+                A.out_h[k] = hN;
+                A.out_hu[k] = huN;
+                A.out_hv[k] = hvN;
+            }
+        }
+    }
+};
+
+// Occupancy target of the stage kernel (waves per SIMD); the VGPR budget follows from it
+// (512 / waves).  Measured on the 8192^2 dam break: 3 waves (134 VGPRs, no spills) beats
+// 4 (128 VGPRs) by ~2%; 5 and 6 spill.  Overridable at build time for tuning sweeps.
+#ifndef SWE_STAGE_WAVES_PER_EU
+#define SWE_STAGE_WAVES_PER_EU 3
+#endif
+
+template <int STAGE, int BC, bool SPEED, bool FAST>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SWE_STAGE_WAVES_PER_EU)))
+void stage_kernel(const GridDev G, const StageArgs A) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
     const int strip = blockIdx.x * kBlockWaves + wave;
@@ -77,151 +191,49 @@ __global__ __launch_bounds__(256) void stage_kernel(const GridDev G, const Stage
     int bad = 0;
 
     if (strip < A.nstrips) {
-        const int ng = G.ng, nx = G.nx, ny = G.ny, s = G.stride;
-        const double half_g = G.half_g, g = G.g, inv_dx = G.inv_dx, inv_dy = G.inv_dy;
-
+        const int ng = G.ng, nx = G.nx, ny = G.ny;
         const int j = ng - 2 + strip * kStripCols + lane;
-        const bool owner = lane >= 2 && lane < kWave - 2 && j < ng + ny;
         bool gcol;
         const int jc = mirror<BC>(j < ng + ny + 2 ? j : ng + ny + 1, ng, ng + ny, gcol);
-
         const int r0 = ng + blockIdx.y * A.tile_rows;
         const int r1 = (r0 + A.tile_rows < ng + nx) ? r0 + A.tile_rows : ng + nx;
-        const int rmax = ng + nx + 1;
 
-        double dt = 0.0;
-        if (STAGE != 0) dt = *A.dt;
+        March<STAGE, BC, SPEED, FAST> m{A, G.stride, ng, nx, ng + nx + 1, r1, j, jc, gcol,
+                                  lane >= 2 && lane < kWave - 2 && j < ng + ny,
+                                  G.half_g, G.g, G.inv_dx, G.inv_dy,
+                                  STAGE != 0 ? *A.dt : 0.0, 0.0, 0};
 
-        auto load = [&](int r) -> Cell {
-            bool gr;
-            const int rs = mirror<BC>(r, ng, ng + nx, gr);
-            return load_cell<BC>(A, rs * s, gr, jc, gcol);
-        };
-
-        // ---- prologue: rows r0-2 .. r0+2, x face r0-1/2
-        const Cell P2 = load(r0 - 2);
-        const Cell P1 = load(r0 - 1);
-        Cell Ac = load(r0);
-        Cell Bc = load(r0 + 1);
-        Cell Nc = load(r0 + 2 <= rmax ? r0 + 2 : rmax);
-
-        double u0h = 0.0, u0u = 0.0, u0v = 0.0;
-        if (STAGE >= 2) {
-            const int k = r0 * s + jc;
-            u0h = A.u0_h[k];
-            u0u = A.u0_hu[k];
-            u0v = A.u0_hv[k];
-        }
+        // ---- prologue: rows r0-2 .. r0+2 and the plus-side flux of face r0-1/2
+        const Cell P2 = m.load(r0 - 2);
+        const Cell P1 = m.load(r0 - 1);
+        Cell C0 = m.load(r0), C1 = m.load(r0 + 1), C2 = m.load(r0 + 2 <= m.rmax ? r0 + 2 : m.rmax);
+        Cell C3;
+        V3 U0{0.0, 0.0, 0.0}, U1{0.0, 0.0, 0.0};
+        if (STAGE >= 2) U0 = m.load_u0(r0);
 
         // x differences d_{r0-3/2}, d_{r0-1/2}, d_{r0+1/2}
-        const double e0 = P1.eta - P2.eta, u0d = P1.hu - P2.hu, v0d = P1.hv - P2.hv;
-        const double e1 = Ac.eta - P1.eta, u1d = Ac.hu - P1.hu, v1d = Ac.hv - P1.hv;
-        double e2 = Bc.eta - Ac.eta, u2d = Bc.hu - Ac.hu, v2d = Bc.hv - Ac.hv;
+        const V3 dm{P1.eta - P2.eta, P1.hu - P2.hu, P1.hv - P2.hv};
+        const V3 d0{C0.eta - P1.eta, C0.hu - P1.hu, C0.hv - P1.hv};
+        V3 D0{C1.eta - C0.eta, C1.hu - C0.hu, C1.hv - C0.hv}, D1;
+        // minus-side reconstruction of row r0-1 at face r0-1/2, slope of row r0
+        const V3 sp{0.5 * minmod(dm.e, d0.e), 0.5 * minmod(dm.u, d0.u), 0.5 * minmod(dm.v, d0.v)};
+        V3 S0{0.5 * minmod(d0.e, D0.e), 0.5 * minmod(d0.u, D0.u), 0.5 * minmod(d0.v, D0.v)}, S1;
+        const FaceFlux f0 = face_flux<FAST>(m.half_g, m.g, P1.eta + sp.e, P1.hu + sp.u, P1.hv + sp.v,
+                                      P1.b, C0.eta - S0.e, C0.hu - S0.u, C0.hv - S0.v, C0.b);
+        V3 F0{f0.fh, f0.fn_p, f0.ft}, F1;
 
-        // minus-side reconstruction of cell r0-1 at face r0-1/2
-        const double sP_e = 0.5 * minmod(e0, e1);
-        const double sP_u = 0.5 * minmod(u0d, u1d);
-        const double sP_v = 0.5 * minmod(v0d, v1d);
-        // slopes of cell r0: plus side at r0-1/2, minus side at r0+1/2
-        double sA_e = 0.5 * minmod(e1, e2);
-        double sA_u = 0.5 * minmod(u1d, u2d);
-        double sA_v = 0.5 * minmod(v1d, v2d);
-
-        FaceFlux fx = face_flux(half_g, g, P1.eta + sP_e, P1.hu + sP_u, P1.hv + sP_v, P1.b,
-                                Ac.eta - sA_e, Ac.hu - sA_u, Ac.hv - sA_v, Ac.b);
-        // plus-side flux of face r-1/2 as seen by cell r
-        double fxp_h = fx.fh, fxp_hu = fx.fn_p, fxp_hv = fx.ft;
-
-        for (int r = r0; r < r1; ++r) {
-            // prefetch row r+3 and U^n at row r+1
-            const int rn = (r + 3 <= rmax) ? r + 3 : rmax;
-            const Cell Nn = load(rn);
-            double n0h = 0.0, n0u = 0.0, n0v = 0.0;
-            if (STAGE >= 2) {
-                const int k = ((r + 1 < r1) ? r + 1 : r) * s + jc;
-                n0h = A.u0_h[k];
-                n0u = A.u0_hu[k];
-                n0v = A.u0_hv[k];
-            }
-
-            // ---- x: slope of cell r+1, face r+1/2
-            const double e3 = Nc.eta - Bc.eta, u3d = Nc.hu - Bc.hu, v3d = Nc.hv - Bc.hv;
-            const double sB_e = 0.5 * minmod(e2, e3);
-            const double sB_u = 0.5 * minmod(u2d, u3d);
-            const double sB_v = 0.5 * minmod(v2d, v3d);
-
-            fx = face_flux(half_g, g, Ac.eta + sA_e, Ac.hu + sA_u, Ac.hv + sA_v, Ac.b,
-                           Bc.eta - sB_e, Bc.hu - sB_u, Bc.hv - sB_v, Bc.b);
-
-            // ---- y: neighbours along the lanes, face j+1/2 in lane j
-            const double dy_e = from_next(Ac.eta) - Ac.eta;
-            const double dy_u = from_next(Ac.hu) - Ac.hu;
-            const double dy_v = from_next(Ac.hv) - Ac.hv;
-            const double sy_e = 0.5 * minmod(from_prev(dy_e), dy_e);
-            const double sy_u = 0.5 * minmod(from_prev(dy_u), dy_u);
-            const double sy_v = 0.5 * minmod(from_prev(dy_v), dy_v);
-            const double pe = from_next(Ac.eta - sy_e);
-            const double pu = from_next(Ac.hu - sy_u);
-            const double pv = from_next(Ac.hv - sy_v);
-            const double pb = from_next(Ac.b);
-            // y faces: normal momentum is hv, tangential hu
-            const FaceFlux fy = face_flux(half_g, g, Ac.eta + sy_e, Ac.hv + sy_v, Ac.hu + sy_u,
-                                          Ac.b, pe, pv, pu, pb);
-            const double fyp_h = from_prev(fy.fh);
-            const double fyp_hu = from_prev(fy.ft);
-            const double fyp_hv = from_prev(fy.fn_p);
-
-            // ---- divergence (omp_kernels.hpp:281-283)
-            const double rh = divergence(inv_dx, inv_dy, fx.fh, fxp_h, fy.fh, fyp_h);
-            const double rhu = divergence(inv_dx, inv_dy, fx.fn_m, fxp_hu, fy.ft, fyp_hu);
-            const double rhv = divergence(inv_dx, inv_dy, fx.ft, fxp_hv, fy.fn_m, fyp_hv);
-
-            if (owner) {
-                const int k = r * s + j;
-                if (STAGE == 0) {
-                    A.out_h[k] = rh;
-                    A.out_hu[k] = rhu;
-                    A.out_hv[k] = rhv;
-                } else {
-                    if (STAGE == 1) {
-                        u0h = Ac.h;
-                        u0u = Ac.hu;
-                        u0v = Ac.hv;
-                    }
-                    double hN = rk_combine<STAGE>(u0h, Ac.h, dt, rh);
-                    double huN = rk_combine<STAGE>(u0u, Ac.hu, dt, rhu);
-                    double hvN = rk_combine<STAGE>(u0v, Ac.hv, dt, rhv);
-                    // enforce_positivity (omp_kernels.hpp:317-325)
-                    if (hN < kFloor) {
-                        hN = kFloor;
-                        huN = 0.0;
-                        hvN = 0.0;
-                    }
-                    if (!isfinite(hN) || !isfinite(huN) || !isfinite(hvN)) bad = 1;
-                    if (SPEED) speed = cell_speed(g, hN, huN, hvN, speed);
-                    A.out_h[k] = hN;
-                    A.out_hu[k] = huN;
-                    A.out_hv[k] = hvN;
-                }
-            }
-
-            // ---- shift the x window
-            fxp_h = fx.fh;
-            fxp_hu = fx.fn_p;
-            fxp_hv = fx.ft;
-            Ac = Bc;
-            Bc = Nc;
-            Nc = Nn;
-            sA_e = sB_e;
-            sA_u = sB_u;
-            sA_v = sB_v;
-            e2 = e3;
-            u2d = u3d;
-            v2d = v3d;
-            u0h = n0h;
-            u0u = n0u;
-            u0v = n0v;
+        // ---- march, unrolled 4x so the register window rotates by renaming
+        for (int r = r0; r < r1; r += 4) {
+            m.row(r, C0, C1, C2, C3, S0, S1, D0, D1, F0, F1, U0, U1);
+            if (r + 1 >= r1) break;
+            m.row(r + 1, C1, C2, C3, C0, S1, S0, D1, D0, F1, F0, U1, U0);
+            if (r + 2 >= r1) break;
+            m.row(r + 2, C2, C3, C0, C1, S0, S1, D0, D1, F0, F1, U0, U1);
+            if (r + 3 >= r1) break;
+            m.row(r + 3, C3, C0, C1, C2, S1, S0, D1, D0, F1, F0, U1, U0);
         }
+        speed = m.speed;
+        bad = m.bad;
     }
 
     if (STAGE != 0) {
@@ -235,9 +247,9 @@ __global__ __launch_bounds__(256) void stage_kernel(const GridDev G, const Stage
         if (lane == 0) red[wave] = speed;
         __syncthreads();
         if (threadIdx.x == 0) {
-            double m = red[0];
-            for (int w = 1; w < kBlockWaves; ++w) m = smax(m, red[w]);
-            A.partial[blockIdx.y * gridDim.x + blockIdx.x] = m;
+            double mx = red[0];
+            for (int w = 1; w < kBlockWaves; ++w) mx = smax(mx, red[w]);
+            A.partial[blockIdx.y * gridDim.x + blockIdx.x] = mx;
         }
     }
 }
@@ -343,23 +355,23 @@ __global__ void fill_double_kernel(double *dst, double v, long n) {
     if (t < n) dst[t] = v;
 }
 
-template <int STAGE, int BC, bool SPEED>
+template <int STAGE, int BC, bool SPEED, bool FAST>
 hipError_t launch_one(const GridDev &G, const StageArgs &A, hipStream_t stream, dim3 grid) {
-    hipLaunchKernelGGL((stage_kernel<STAGE, BC, SPEED>), grid, dim3(kWave * kBlockWaves), 0,
+    hipLaunchKernelGGL((stage_kernel<STAGE, BC, SPEED, FAST>), grid, dim3(kWave * kBlockWaves), 0,
                        stream, G, A);
     return hipGetLastError();
 }
 
-template <int BC>
+template <int BC, bool FAST>
 hipError_t launch_bc(int stage, bool speed, const GridDev &G, const StageArgs &A,
                      hipStream_t stream, dim3 grid) {
     switch (stage) {
-        case 0: return launch_one<0, BC, false>(G, A, stream, grid);
-        case 1: return launch_one<1, BC, false>(G, A, stream, grid);
-        case 2: return launch_one<2, BC, false>(G, A, stream, grid);
+        case 0: return launch_one<0, BC, false, FAST>(G, A, stream, grid);
+        case 1: return launch_one<1, BC, false, FAST>(G, A, stream, grid);
+        case 2: return launch_one<2, BC, false, FAST>(G, A, stream, grid);
         case 3:
-            return speed ? launch_one<3, BC, true>(G, A, stream, grid)
-                         : launch_one<3, BC, false>(G, A, stream, grid);
+            return speed ? launch_one<3, BC, true, FAST>(G, A, stream, grid)
+                         : launch_one<3, BC, false, FAST>(G, A, stream, grid);
         default: return hipErrorInvalidValue;
     }
 }
@@ -371,13 +383,16 @@ dim3 stage_grid(const GridDev &G, int tile_rows) {
     return dim3((nstrips + kBlockWaves - 1) / kBlockWaves, (G.nx + tile_rows - 1) / tile_rows, 1);
 }
 
-hipError_t launch_stage(int stage, int bc, bool speed, const GridDev &G, const StageArgs &A,
-                        hipStream_t stream, dim3 *grid_out) {
+hipError_t launch_stage(int stage, int bc, bool speed, bool fast, const GridDev &G,
+                        const StageArgs &A, hipStream_t stream, dim3 *grid_out) {
     if (A.tile_rows < 1 || A.nstrips != strips_for(G.ny)) return hipErrorInvalidValue;
     const dim3 grid = stage_grid(G, A.tile_rows);
     if (grid_out) *grid_out = grid;
-    return bc == 0 ? launch_bc<0>(stage, speed, G, A, stream, grid)
-                   : launch_bc<1>(stage, speed, G, A, stream, grid);
+    if (fast)
+        return bc == 0 ? launch_bc<0, true>(stage, speed, G, A, stream, grid)
+                       : launch_bc<1, true>(stage, speed, G, A, stream, grid);
+    return bc == 0 ? launch_bc<0, false>(stage, speed, G, A, stream, grid)
+                   : launch_bc<1, false>(stage, speed, G, A, stream, grid);
 }
 
 hipError_t launch_speed_reduce(const GridDev &G, const double *h, const double *hu,

@@ -13,7 +13,8 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libswe_hip.so")
+# SWE_HIP_LIB overrides the library path (used to time alternative builds side by side).
+LIB_PATH = os.environ.get("SWE_HIP_LIB") or os.path.join(HERE, "libswe_hip.so")
 
 SWE_OK = 0
 SWE_ERR_INVALID = 1
@@ -25,6 +26,9 @@ SWE_ERR_NO_DEVICE = 6
 
 BC_REFLECTING = 0
 BC_TRANSMISSIVE = 1
+
+PRECISION_EXACT = 0   # bitwise the reference's arithmetic
+PRECISION_FAST = 1    # reciprocal-based velocities, few-ulp differences per step
 
 G_REF = 9.80665  # reference include/constants.hpp:4
 
@@ -64,6 +68,8 @@ _SIGS = {
     "swe_hip_rhs": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "swe_hip_set_tile_rows": (C.c_int, [C.c_void_p, C.c_int]),
     "swe_hip_get_tile_rows": (C.c_int, [C.c_void_p]),
+    "swe_hip_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
+    "swe_hip_get_precision": (C.c_int, [C.c_void_p]),
     "swe_hip_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
     "swe_hip_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "swe_hip_time_steps": (C.c_int, [C.c_void_p, C.c_double, C.c_int64, _dp, _dp]),
@@ -196,6 +202,14 @@ class HipSolver:
     @tile_rows.setter
     def tile_rows(self, v):
         self._check(lib().swe_hip_set_tile_rows(self._ctx, int(v)))
+
+    @property
+    def precision(self):
+        return lib().swe_hip_get_precision(self._ctx)
+
+    @precision.setter
+    def precision(self, mode):
+        self._check(lib().swe_hip_set_precision(self._ctx, int(mode)))
 
     def set_graph(self, enabled: bool):
         self._check(lib().swe_hip_set_graph(self._ctx, int(bool(enabled))))
