@@ -247,6 +247,7 @@ int swe_hip_create(const swe_grid_desc *d, int device, swe_hip_ctx **out) {
     c->G.inv_dy = 1.0 / d->dy;
     c->G.g = d->g;
     c->G.half_g = 0.5 * d->g;
+    c->G.split_ok = (d->g >= 0x1p-10 && d->g <= 0x1p10) ? 1 : 0;
     c->n_total = (size_t)(nxt * nyt);
     c->tile_rows = default_tile_rows(c->G);
 

@@ -53,6 +53,68 @@ __device__ __forceinline__ double recip(double d) {
     return fma(r1, fma(-d, r1, 1.0), r1);
 }
 
+// ---- IEEE division and square root, split for operands in a guarded range -------------
+//
+// On gfx950 an f64 division n/d is the sequence
+//   ds = div_scale(d, d, n); r = rcp(ds); ns = div_scale(n, d, n)
+//   e = fma(-ds, r, 1); r = fma(r, e, r); e = fma(-ds, r, 1); r = fma(r, e, r)
+//   q = ns * r; rem = fma(-ds, q, ns); q = div_fmas(rem, r, q); n/d = div_fixup(q, d, n)
+// and sqrt(x) is rsq + six fma/mul Newton steps, bracketed by a rescale for x < 2^-767 and
+// a select for x in {+-0, +inf}.  When n and d are normal, d > 0, n/d is normal and the
+// exponents are far from the limits (|n| >= 2^-969, exponent(n) - exponent(d) < 768),
+// div_scale returns its operand unchanged with VCC = 0, div_fmas is a plain fma and
+// div_fixup only applies the sign rule sign(n) ^ sign(d); sqrt's rescale is the identity
+// for x >= 2^-767.  The split forms below compute exactly those reduced sequences, so the
+// results are the same bits as `/` and sqrt() -- and the denominator part (rcp + 4 fma) is
+// shared by the two divisions that have the same depth as denominator.  Callers use them
+// only under the range guard of the stage kernel (every cell value 0 or of magnitude in
+// [2^-240, 2^190]), which bounds every numerator, denominator and sqrt argument of a face
+// well inside those conditions.
+__device__ __forceinline__ double div_denominator(double d) {
+    double r = __builtin_amdgcn_rcp(d);
+    double e = fma(-d, r, 1.0);
+    r = fma(r, e, r);
+    e = fma(-d, r, 1.0);
+    return fma(r, e, r);
+}
+
+// n / d given r = div_denominator(d), for d > 0.
+__device__ __forceinline__ double div_numerator(double n, double d, double r) {
+    const double q = n * r;
+    const double rem = fma(-d, q, n);
+    return copysign(fma(rem, r, q), n);
+}
+
+// sqrt(x) for x = 0 or x >= 2^-767 (finite).
+__device__ __forceinline__ double sqrt_unscaled(double x) {
+    const double y = __builtin_amdgcn_rsq(x);
+    double g = x * y;
+    double h = y * 0.5;
+    const double r = fma(-h, g, 0.5);
+    g = fma(g, r, g);
+    h = fma(h, r, h);
+    double d = fma(-g, g, x);
+    g = fma(d, h, g);
+    d = fma(-g, g, x);
+    g = fma(d, h, g);
+    return x == 0.0 ? x : g;
+}
+
+// The range of cell values under which the split forms are exact (see above): zero, or a
+// magnitude in [2^-240, 2^190].  NaN, infinities and denormals fail it.
+__device__ __forceinline__ bool split_ok(double v) {
+    const double a = fabs(v);
+    return (a >= 0x1p-240 && a <= 0x1p190) || v == 0.0;
+}
+
+// Division and square root in the exact mode: the split forms when SPLIT (range proven),
+// the compiler's full sequences otherwise.
+template <bool SPLIT>
+__device__ __forceinline__ double ieee_sqrt(double x) {
+    if constexpr (SPLIT) return sqrt_unscaled(x);
+    return sqrt(x);
+}
+
 // HLL on the hydrostatically reconstructed states (omp_kernels.hpp:50-86 / :88-124).
 //
 // FAST = false: the reference arithmetic, every division IEEE (bitwise parity).
@@ -60,7 +122,7 @@ __device__ __forceinline__ double recip(double d) {
 //   (vnL.. = the face velocities before the hydrostatic depth change, so hs*u/hs -> u, a
 //   1-ulp difference) and 1/(sR - sL) is a refined reciprocal.  Results then differ from
 //   the reference by a few ulps per step, which the tests bound after N steps.
-template <bool FAST>
+template <bool FAST, bool SPLIT>
 __device__ __forceinline__ void hll(double half_g, double g, double hL, double nL, double tL,
                                     double hR, double nR, double tR, double vnL, double vtL,
                                     double vnR, double vtR, double &fh, double &fn,
@@ -71,6 +133,20 @@ __device__ __forceinline__ void hll(double half_g, double g, double hL, double n
         utL = hL > kEps ? vtL : 0.0;
         unR = hR > kEps ? vnR : 0.0;
         utR = hR > kEps ? vtR : 0.0;
+    } else if constexpr (SPLIT) {
+        const double rL = div_denominator(hL), rR = div_denominator(hR);
+        const double qnL = div_numerator(nL, hL, rL), qtL = div_numerator(tL, hL, rL);
+        const double qnR = div_numerator(nR, hR, rR), qtR = div_numerator(tR, hR, rR);
+        unL = hL > kEps ? qnL : 0.0;
+        utL = hL > kEps ? qtL : 0.0;
+        unR = hR > kEps ? qnR : 0.0;
+        utR = hR > kEps ? qtR : 0.0;
+    } else if (__all(hL > kEps && hR > kEps)) {
+        // whole wave wet: safe_velocity takes its division branch in every lane
+        unL = nL / hL;
+        utL = tL / hL;
+        unR = nR / hR;
+        utR = tR / hR;
     } else {
         unL = safe_velocity(nL, hL);
         utL = safe_velocity(tL, hL);
@@ -78,8 +154,8 @@ __device__ __forceinline__ void hll(double half_g, double g, double hL, double n
         utR = safe_velocity(tR, hR);
     }
 
-    const double cL = sqrt(g * hL);
-    const double cR = sqrt(g * hR);
+    const double cL = ieee_sqrt<SPLIT>(g * hL);
+    const double cR = ieee_sqrt<SPLIT>(g * hR);
 
     // fmin/fmax instead of std::min/max: they differ only in the sign of a zero result or
     // on NaN, and a zero sL (sR) takes the sL >= 0 (sR <= 0) branch where its value is unused.
@@ -93,7 +169,25 @@ __device__ __forceinline__ void hll(double half_g, double g, double hL, double n
     const double FR_n = nR * unR + half_g * hR * hR;
     const double FR_t = nR * utR;
 
-    if (sL >= 0.0) {
+    // Star (subcritical) branch; when every lane of the wave is subcritical the selects of
+    // the one-sided branches are skipped (same values, fewer instructions).
+    auto star = [&](double &h_, double &n_, double &t_) {
+        double inv;
+        if constexpr (FAST) {
+            inv = recip(sR - sL);
+        } else if constexpr (SPLIT) {
+            const double d = sR - sL;
+            inv = div_numerator(1.0, d, div_denominator(d));
+        } else {
+            inv = 1.0 / (sR - sL);
+        }
+        h_ = (sR * FL_h - sL * FR_h + sL * sR * (hR - hL)) * inv;
+        n_ = (sR * FL_n - sL * FR_n + sL * sR * (nR - nL)) * inv;
+        t_ = (sR * FL_t - sL * FR_t + sL * sR * (tR - tL)) * inv;
+    };
+    if (__all(sL < 0.0 && sR > 0.0)) {
+        star(fh, fn, ft);
+    } else if (sL >= 0.0) {
         fh = FL_h;
         fn = FL_n;
         ft = FL_t;
@@ -102,16 +196,13 @@ __device__ __forceinline__ void hll(double half_g, double g, double hL, double n
         fn = FR_n;
         ft = FR_t;
     } else {
-        const double inv = FAST ? recip(sR - sL) : 1.0 / (sR - sL);
-        fh = (sR * FL_h - sL * FR_h + sL * sR * (hR - hL)) * inv;
-        fn = (sR * FL_n - sL * FR_n + sL * sR * (nR - nL)) * inv;
-        ft = (sR * FL_t - sL * FR_t + sL * sR * (tR - tL)) * inv;
+        star(fh, fn, ft);
     }
 }
 
 // One face from the MUSCL-reconstructed minus/plus values of (eta, n, t) and the cell
 // bottoms on either side (omp_kernels.hpp:154-189 for x, :222-257 for y).
-template <bool FAST>
+template <bool FAST, bool SPLIT>
 __device__ __forceinline__ FaceFlux face_flux(double half_g, double g, double eta_m, double n_m,
                                               double t_m, double b_m, double eta_p, double n_p,
                                               double t_p, double b_p) {
@@ -130,6 +221,20 @@ __device__ __forceinline__ FaceFlux face_flux(double half_g, double g, double et
         ut_m = h_m > kEps ? t_m * rm : 0.0;
         un_p = h_p > kEps ? n_p * rp : 0.0;
         ut_p = h_p > kEps ? t_p * rp : 0.0;
+    } else if constexpr (SPLIT) {
+        const double rm = div_denominator(h_m), rp = div_denominator(h_p);
+        const double qnm = div_numerator(n_m, h_m, rm), qtm = div_numerator(t_m, h_m, rm);
+        const double qnp = div_numerator(n_p, h_p, rp), qtp = div_numerator(t_p, h_p, rp);
+        un_m = h_m > kEps ? qnm : 0.0;
+        ut_m = h_m > kEps ? qtm : 0.0;
+        un_p = h_p > kEps ? qnp : 0.0;
+        ut_p = h_p > kEps ? qtp : 0.0;
+    } else if (__all(h_m > kEps && h_p > kEps)) {
+        // whole wave wet: safe_velocity takes its division branch in every lane
+        un_m = n_m / h_m;
+        ut_m = t_m / h_m;
+        un_p = n_p / h_p;
+        ut_p = t_p / h_p;
     } else {
         un_m = safe_velocity(n_m, h_m);
         ut_m = safe_velocity(t_m, h_m);
@@ -141,8 +246,8 @@ __device__ __forceinline__ FaceFlux face_flux(double half_g, double g, double et
     const double hs_p = positive_height(eta_p - b_star);
 
     double fh, fn, ft;
-    hll<FAST>(half_g, g, hs_m, hs_m * un_m, hs_m * ut_m, hs_p, hs_p * un_p, hs_p * ut_p, un_m,
-              ut_m, un_p, ut_p, fh, fn, ft);
+    hll<FAST, SPLIT>(half_g, g, hs_m, hs_m * un_m, hs_m * ut_m, hs_p, hs_p * un_p, hs_p * ut_p,
+                     un_m, ut_m, un_p, ut_p, fh, fn, ft);
 
     const double corr_m = half_g * (h_m * h_m - hs_m * hs_m);
     const double corr_p = half_g * (h_p * h_p - hs_p * hs_p);

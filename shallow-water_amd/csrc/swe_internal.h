@@ -11,6 +11,7 @@ namespace swe {
 struct GridDev {
     int nx, ny, ng, stride;
     double dx, dy, inv_dx, inv_dy, g, half_g;
+    int split_ok;  // g in [2^-10, 2^10]: the split division/sqrt forms may be used
 };
 
 // Arguments of one fused RK stage launch.

@@ -61,15 +61,30 @@ __device__ __forceinline__ int mirror(int r, int lo, int hi, bool &ghost) {
     return r;
 }
 
+// The row offset is wave-uniform and the column is fixed per lane, so every access is a
+// scalar row base plus a constant 32-bit per-lane byte offset: the global_load/store
+// "saddr + voffset" form, with no per-row 64-bit address arithmetic on the vector ALUs.
+__device__ __forceinline__ const double *row_ptr(const double *base, int row_off) {
+    return base + __builtin_amdgcn_readfirstlane(row_off);
+}
+__device__ __forceinline__ double *row_ptr(double *base, int row_off) {
+    return base + __builtin_amdgcn_readfirstlane(row_off);
+}
+__device__ __forceinline__ double ld(const double *row, unsigned byte_off) {
+    return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(row) + byte_off);
+}
+__device__ __forceinline__ void st(double *row, unsigned byte_off, double v) {
+    *reinterpret_cast<double *>(reinterpret_cast<char *>(row) + byte_off) = v;
+}
+
 template <int BC>
 __device__ __forceinline__ Cell load_cell(const StageArgs &A, int row_off, bool ghost_row,
-                                          int jc, bool ghost_col) {
-    const int k = row_off + jc;
+                                          unsigned jc_bytes, bool ghost_col) {
     Cell c;
-    c.h = A.in_h[k];
-    c.hu = A.in_hu[k];
-    c.hv = A.in_hv[k];
-    c.b = A.B[k];
+    c.h = ld(row_ptr(A.in_h, row_off), jc_bytes);
+    c.hu = ld(row_ptr(A.in_hu, row_off), jc_bytes);
+    c.hv = ld(row_ptr(A.in_hv, row_off), jc_bytes);
+    c.b = ld(row_ptr(A.B, row_off), jc_bytes);
     if (BC == 0) {
         if (ghost_row) c.hu = -c.hu;
         if (ghost_col) c.hv = -c.hv;
@@ -90,21 +105,27 @@ struct V3 {
 template <int STAGE, int BC, bool SPEED, bool FAST>
 struct March {
     const StageArgs &A;
-    int s, ng, nx, rmax, r1, j, jc;
+    int s, ng, nx, rmax, r1;
+    unsigned j_bytes, jc_bytes;  // own column / source column, in bytes from the row start
     bool gcol, owner;
     double half_g, g, inv_dx, inv_dy, dt;
     double speed;
     int bad;
+    // Wave-uniform: every cell loaded so far by this wave satisfies split_ok, so the faces
+    // may use the split IEEE division/sqrt forms (swe_face.h).  Sticky once false.
+    bool split;
 
-    __device__ __forceinline__ Cell load(int r) const {
+    __device__ __forceinline__ Cell load(int r) {
         bool gr;
         const int rs = mirror<BC>(r, ng, ng + nx, gr);
-        return load_cell<BC>(A, rs * s, gr, jc, gcol);
+        const Cell c = load_cell<BC>(A, rs * s, gr, jc_bytes, gcol);
+        split = split && __all(split_ok(c.h) && split_ok(c.hu) && split_ok(c.hv) && split_ok(c.b));
+        return c;
     }
 
     __device__ __forceinline__ V3 load_u0(int r) const {
-        const int k = r * s + jc;
-        return V3{A.u0_h[k], A.u0_hu[k], A.u0_hv[k]};
+        return V3{ld(row_ptr(A.u0_h, r * s), jc_bytes), ld(row_ptr(A.u0_hu, r * s), jc_bytes),
+                  ld(row_ptr(A.u0_hv, r * s), jc_bytes)};
     }
 
     __device__ __forceinline__ void row(int r, const Cell &a, const Cell &b, const Cell &n,
@@ -113,11 +134,21 @@ struct March {
         // prefetch row r+3 and U^n at row r+1
         nn = load(r + 3 <= rmax ? r + 3 : rmax);
         if (STAGE >= 2) u0n = load_u0(r + 1 < r1 ? r + 1 : r);
+        // FAST has its own reciprocal arithmetic; only the exact mode has a split variant.
+        if (!FAST && split)
+            row_body<true>(r, a, b, n, sa, sb, d2, d3, fxp, fxn, u0);
+        else
+            row_body<false>(r, a, b, n, sa, sb, d2, d3, fxp, fxn, u0);
+    }
 
+    template <bool SPLIT>
+    __device__ __forceinline__ void row_body(int r, const Cell &a, const Cell &b, const Cell &n,
+                                             const V3 &sa, V3 &sb, const V3 &d2, V3 &d3,
+                                             const V3 &fxp, V3 &fxn, const V3 &u0) {
         // ---- x: slope of row r+1, face r+1/2
         d3 = V3{n.eta - b.eta, n.hu - b.hu, n.hv - b.hv};
         sb = V3{0.5 * minmod(d2.e, d3.e), 0.5 * minmod(d2.u, d3.u), 0.5 * minmod(d2.v, d3.v)};
-        const FaceFlux fx = face_flux<FAST>(half_g, g, a.eta + sa.e, a.hu + sa.u, a.hv + sa.v, a.b,
+        const FaceFlux fx = face_flux<FAST, SPLIT>(half_g, g, a.eta + sa.e, a.hu + sa.u, a.hv + sa.v, a.b,
                                       b.eta - sb.e, b.hu - sb.u, b.hv - sb.v, b.b);
         fxn = V3{fx.fh, fx.fn_p, fx.ft};
 
@@ -133,7 +164,7 @@ struct March {
         const double pv = from_next(a.hv - sy_v);
         const double pb = from_next(a.b);
         // y faces: normal momentum is hv, tangential hu
-        const FaceFlux fy = face_flux<FAST>(half_g, g, a.eta + sy_e, a.hv + sy_v, a.hu + sy_u, a.b, pe,
+        const FaceFlux fy = face_flux<FAST, SPLIT>(half_g, g, a.eta + sy_e, a.hv + sy_v, a.hu + sy_u, a.b, pe,
                                       pv, pu, pb);
         const double fyp_h = from_prev(fy.fh);
         const double fyp_hu = from_prev(fy.ft);
@@ -145,11 +176,11 @@ struct March {
         const double rhv = divergence(inv_dx, inv_dy, fx.ft, fxp.v, fy.fn_m, fyp_hv);
 
         if (owner) {
-            const int k = r * s + j;
+            const int ro = r * s;
             if (STAGE == 0) {
-                A.out_h[k] = rh;
-                A.out_hu[k] = rhu;
-                A.out_hv[k] = rhv;
+                st(row_ptr(A.out_h, ro), j_bytes, rh);
+                st(row_ptr(A.out_hu, ro), j_bytes, rhu);
+                st(row_ptr(A.out_hv, ro), j_bytes, rhv);
             } else {
                 const double s0h = STAGE == 1 ? a.h : u0.e;
                 const double s0u = STAGE == 1 ? a.hu : u0.u;
@@ -165,9 +196,9 @@ struct March {
                 }
                 if (!isfinite(hN) || !isfinite(huN) || !isfinite(hvN)) bad = 1;
                 if (SPEED) speed = cell_speed(g, hN, huN, hvN, speed);
-                A.out_h[k] = hN;
-                A.out_hu[k] = huN;
-                A.out_hv[k] = hvN;
+                st(row_ptr(A.out_h, ro), j_bytes, hN);
+                st(row_ptr(A.out_hu, ro), j_bytes, huN);
+                st(row_ptr(A.out_hv, ro), j_bytes, hvN);
             }
         }
     }
@@ -198,10 +229,11 @@ void stage_kernel(const GridDev G, const StageArgs A) {
         const int r0 = ng + blockIdx.y * A.tile_rows;
         const int r1 = (r0 + A.tile_rows < ng + nx) ? r0 + A.tile_rows : ng + nx;
 
-        March<STAGE, BC, SPEED, FAST> m{A, G.stride, ng, nx, ng + nx + 1, r1, j, jc, gcol,
-                                  lane >= 2 && lane < kWave - 2 && j < ng + ny,
-                                  G.half_g, G.g, G.inv_dx, G.inv_dy,
-                                  STAGE != 0 ? *A.dt : 0.0, 0.0, 0};
+        March<STAGE, BC, SPEED, FAST> m{A, G.stride, ng, nx, ng + nx + 1, r1,
+                                        (unsigned)j * 8u, (unsigned)jc * 8u, gcol,
+                                        lane >= 2 && lane < kWave - 2 && j < ng + ny,
+                                        G.half_g, G.g, G.inv_dx, G.inv_dy,
+                                        STAGE != 0 ? *A.dt : 0.0, 0.0, 0, G.split_ok != 0};
 
         // ---- prologue: rows r0-2 .. r0+2 and the plus-side flux of face r0-1/2
         const Cell P2 = m.load(r0 - 2);
@@ -218,8 +250,13 @@ void stage_kernel(const GridDev G, const StageArgs A) {
         // minus-side reconstruction of row r0-1 at face r0-1/2, slope of row r0
         const V3 sp{0.5 * minmod(dm.e, d0.e), 0.5 * minmod(dm.u, d0.u), 0.5 * minmod(dm.v, d0.v)};
         V3 S0{0.5 * minmod(d0.e, D0.e), 0.5 * minmod(d0.u, D0.u), 0.5 * minmod(d0.v, D0.v)}, S1;
-        const FaceFlux f0 = face_flux<FAST>(m.half_g, m.g, P1.eta + sp.e, P1.hu + sp.u, P1.hv + sp.v,
-                                      P1.b, C0.eta - S0.e, C0.hu - S0.u, C0.hv - S0.v, C0.b);
+        const FaceFlux f0 =
+            (!FAST && m.split) ? face_flux<FAST, true>(m.half_g, m.g, P1.eta + sp.e, P1.hu + sp.u,
+                                            P1.hv + sp.v, P1.b, C0.eta - S0.e, C0.hu - S0.u,
+                                            C0.hv - S0.v, C0.b)
+                    : face_flux<FAST, false>(m.half_g, m.g, P1.eta + sp.e, P1.hu + sp.u,
+                                             P1.hv + sp.v, P1.b, C0.eta - S0.e, C0.hu - S0.u,
+                                             C0.hv - S0.v, C0.b);
         V3 F0{f0.fh, f0.fn_p, f0.ft}, F1;
 
         // ---- march, unrolled 4x so the register window rotates by renaming

@@ -166,6 +166,37 @@ def test_adaptive_vs_oracle(hip, bc):
         assert_bitwise(g_, w_, f"adaptive bc={bc} {comp}")
 
 
+@pytest.mark.parametrize("case", ["tiny_values", "huge_values", "denormals"])
+def test_split_guard_fallback(hip, case):
+    """Cells outside [2^-240, 2^190] (and not 0) switch their waves to the full IEEE division
+    and sqrt sequences; results stay bitwise equal either way."""
+    c = W.random_state(90, 140, seed=31)
+    h, hu, hv, B = c["h"], c["hu"], c["hv"], c["B"]
+    if case == "tiny_values":
+        hu[10:14, 20:70] = 1e-300
+        hv[40:42, 100:130] = -3e-250
+        B[60:63, 5:15] = 1e-290
+    elif case == "huge_values":
+        hv[30:33, 50:60] = 1e60   # beyond 2^190? no: 1e60 < 2^190 ~ 1.6e57 is false -> huge
+        B[70, 70:90] = 1e58
+    else:
+        hu[20:25, 80:90] = 5e-320
+        hv[50:52, 10:40] = -1e-310
+    gr, h0, hu0, hv0, B0 = oracle_state(c)
+    want = oracle.rhs(gr, h0.copy(), hu0.copy(), hv0.copy(), B0)
+    with gpu_solver(hip, c) as s:
+        got = s.rhs()
+        dt, _ = s.stable_dt(0.2)
+        s.step(dt, 2)
+        got_state = s.download()
+    for comp, g_, w_ in zip(("h", "hu", "hv"), got, want):
+        assert_bitwise(g_, w_, f"{case} rhs_{comp}")
+    rc, _, _ = oracle.run(gr, h0, hu0, hv0, B0, 2, dt=dt)
+    if rc == 0:
+        for comp, g_, w_ in zip(("h", "hu", "hv"), got_state, (h0, hu0, hv0)):
+            assert_bitwise(g_, w_, f"{case} 2 steps {comp}")
+
+
 @pytest.mark.parametrize("tile_rows", [1, 3, 16, 64, 1000])
 def test_tile_rows_invariant(hip, tile_rows):
     c = W.lake_wet_dry(97, seed=6, eta0=2.0)
