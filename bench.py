@@ -84,9 +84,6 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--event-steps", type=int, default=20)
-    ap.add_argument("--calib-copy", action="store_true",
-                    help="also run a device-to-device copy of the three state arrays "
-                         "(known size) for hardware-counter calibration")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -110,17 +107,6 @@ def main():
     dt, max_speed = s.stable_dt(0.45)
     m0 = mass(case["h"], case["ng"])
     log(f"[bench] dt={dt!r} max_speed={max_speed!r} tile_rows={s.tile_rows}")
-
-    if args.calib_copy:
-        import torch
-
-        # a plain device copy kernel of known size (reads and writes n_total*8 bytes)
-        src = torch.from_numpy(case["h"]).to("cuda")
-        dst = torch.empty_like(src)
-        dst.copy_(src)
-        torch.cuda.synchronize()
-        log(f"[bench] calibration copy kernel: {src.numel() * 8} bytes read and written")
-        del src, dst
 
     # warmup (also builds the step graph)
     s.step(dt, max(args.warmup, 1))

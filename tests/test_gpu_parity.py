@@ -177,7 +177,7 @@ def test_split_guard_fallback(hip, case):
         hv[40:42, 100:130] = -3e-250
         B[60:63, 5:15] = 1e-290
     elif case == "huge_values":
-        hv[30:33, 50:60] = 1e60   # beyond 2^190? no: 1e60 < 2^190 ~ 1.6e57 is false -> huge
+        hv[30:33, 50:60] = 1e60   # above 2^190 (~1.6e57)
         B[70, 70:90] = 1e58
     else:
         hu[20:25, 80:90] = 5e-320
