@@ -74,6 +74,69 @@ def cpu_baseline(case, steps, dt):
     return info, (h, hu, hv)
 
 
+def ensemble_bench(steps, base_steps, check_every=32):
+    """Config 4: 512 instances, sides log-uniform in [64, 1024], per-instance CFL 0.4 dt,
+    advanced together (one tile work-list launch per stage) vs one context per instance
+    (each instance's steps graph-replayed on its own).  Rates in Gcell-updates/s."""
+    import swe_hip
+    from swe_hip import workloads as W
+
+    sides = W.ensemble_sides(512)
+    cases = [W.ensemble_member(i, int(n)) for i, n in enumerate(sides)]
+    cells = int(sum(int(n) * int(n) for n in sides))
+    b = swe_hip.HipBatch(cases)
+    for i, c in enumerate(cases):
+        b.upload(i, c["h"], c["hu"], c["hv"], c["B"])
+    b.stable_dt(0.4)
+    tile_rows, waves = b.autotune()
+    b.step_adaptive(0.4, 2)   # warmup: graph capture
+    for i, c in enumerate(cases):
+        b.upload(i, c["h"], c["hu"], c["hv"], c["B"])
+    t0 = time.perf_counter()
+    b.step_adaptive(0.4, steps)
+    el_b = time.perf_counter() - t0
+    rate_b = cells * steps / el_b / 1e9
+    log(f"[bench] ensemble batched: {steps} steps {el_b:.3f} s -> {rate_b:.2f} Gcell/s "
+        f"(tile_rows {tile_rows}, waves/CU {waves})")
+
+    # per-instance launches on the same initial states, fewer steps (a rate)
+    solvers = []
+    for c in cases:
+        s = swe_hip.HipSolver.from_case(c)
+        s.upload(c["h"], c["hu"], c["hv"], c["B"])
+        s.step_adaptive(0.4, 2)   # warmup / graph capture
+        s.upload(c["h"], c["hu"], c["hv"], c["B"])
+        solvers.append(s)
+    t0 = time.perf_counter()
+    for s in solvers:
+        s.step_adaptive(0.4, base_steps)
+    el_s = time.perf_counter() - t0
+    rate_s = cells * base_steps / el_s / 1e9
+    log(f"[bench] ensemble per-instance: {base_steps} steps {el_s:.3f} s -> {rate_s:.3f} Gcell/s")
+
+    # same number of steps in the batch from the same states; compare every check_every-th
+    # instance bitwise
+    for i, c in enumerate(cases):
+        b.upload(i, c["h"], c["hu"], c["hv"], c["B"])
+    b.step_adaptive(0.4, base_steps)
+    same = True
+    for i in range(0, len(cases), check_every):
+        got = b.download(i)
+        want = solvers[i].download()
+        same &= all(np.array_equal(x.view(np.uint64), y.view(np.uint64)) for x, y in zip(got, want))
+    for s in solvers:
+        s.close()
+    b.close()
+    return {
+        "workload": "512 instances, sides log-uniform [64,1024] (seed 4), 8-bump bathymetry, "
+                    "Gaussian surface, reflecting, per-instance CFL 0.4",
+        "cells": cells, "batched_steps": steps, "batched_gcell_per_s": round(rate_b, 3),
+        "per_instance_steps": base_steps, "per_instance_gcell_per_s": round(rate_s, 4),
+        "speedup": round(rate_b / rate_s, 2), "tile_rows": tile_rows, "waves_per_cu": waves,
+        "bitwise_vs_per_instance": bool(same),
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -84,6 +147,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--event-steps", type=int, default=20)
+    ap.add_argument("--no-ensemble", action="store_true")
+    ap.add_argument("--ensemble-steps", type=int, default=500)
+    ap.add_argument("--ensemble-base-steps", type=int, default=30)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -186,6 +252,8 @@ def main():
         except MemoryError as e:
             out["cpu_baseline"] = None
             log(f"[bench] CPU baseline skipped: {e}")
+    if not args.no_ensemble:
+        out["batched"] = ensemble_bench(args.ensemble_steps, args.ensemble_base_steps)
     print(json.dumps(out), flush=True)
 
 

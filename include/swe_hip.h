@@ -141,6 +141,49 @@ int swe_hip_time_steps(swe_hip_ctx *ctx, double dt, int64_t nsteps, double *stag
 /* Wait for all work on ctx's stream. */
 int swe_hip_sync(swe_hip_ctx *ctx);
 
+/* ---- Batched ensemble --------------------------------------------------------------------
+ * No counterpart in the reference (which runs one grid per process): `count` independent
+ * grids of uneven size advanced together -- every step is three launches over a tile
+ * work-list covering all instances, each instance with its own dt.  All descriptors must
+ * have ng = 2 and share g and bc_kind.  Arrays passed per instance use that instance's
+ * reference layout ((nx+4)*(ny+4) doubles). */
+typedef struct swe_hip_batch swe_hip_batch;
+
+int swe_hip_batch_create(const swe_grid_desc *descs, int32_t count, int device,
+                         swe_hip_batch **out);
+void swe_hip_batch_destroy(swe_hip_batch *batch);
+const char *swe_hip_batch_last_error(const swe_hip_batch *batch);
+int swe_hip_batch_count(const swe_hip_batch *batch);
+
+/* Per-instance host transfers (ghosts rebuilt as in swe_hip_upload / swe_hip_download). */
+int swe_hip_batch_upload(swe_hip_batch *batch, int32_t i, const double *h, const double *hu,
+                         const double *hv, const double *b);
+int swe_hip_batch_download(swe_hip_batch *batch, int32_t i, double *h, double *hu, double *hv);
+
+/* Fixed per-instance dt (count values) for swe_hip_batch_step. */
+int swe_hip_batch_set_dt(swe_hip_batch *batch, const double *dt);
+/* compute_stable_dt of every instance's current state into dt_out[count] (and into the
+ * per-instance dt used by swe_hip_batch_step). */
+int swe_hip_batch_stable_dt(swe_hip_batch *batch, double cfl, double *dt_out);
+/* nsteps fixed-dt steps of every instance. */
+int swe_hip_batch_step(swe_hip_batch *batch, int64_t nsteps);
+/* nsteps steps with per-instance dt = compute_stable_dt(U_i^n, cfl) before each step (a
+ * segmented max reduction on the device); dt_hist (nsteps x count, row-major) may be NULL. */
+int swe_hip_batch_step_adaptive(swe_hip_batch *batch, double cfl, int64_t nsteps,
+                                double *dt_hist);
+/* Per-instance non-finite flags of the last step call (count ints). */
+int swe_hip_batch_nonfinite(swe_hip_batch *batch, int32_t *flags);
+
+/* Launch shape: interior rows per tile and persistent waves per CU (0 = one tile per wave).
+ * swe_hip_batch_autotune times candidates on the current state without modifying it and
+ * keeps the fastest. */
+int swe_hip_batch_set_launch(swe_hip_batch *batch, int tile_rows, int waves_per_cu);
+int swe_hip_batch_get_launch(const swe_hip_batch *batch, int *tile_rows, int *waves_per_cu);
+int swe_hip_batch_autotune(swe_hip_batch *batch, int *tile_rows, int *waves_per_cu);
+int swe_hip_batch_set_precision(swe_hip_batch *batch, int mode);
+int swe_hip_batch_set_graph(swe_hip_batch *batch, int enabled);
+int swe_hip_batch_sync(swe_hip_batch *batch);
+
 #ifdef __cplusplus
 }
 #endif

@@ -34,7 +34,7 @@ constexpr int kWave = 64;
 constexpr int kStripCols = kWave - 4;  // output columns per wave (2-cell halo each side)
 constexpr int kBlockWaves = 4;
 
-inline int strips_for(int ny) { return (ny + kStripCols - 1) / kStripCols; }
+__host__ __device__ inline int strips_for(int ny) { return (ny + kStripCols - 1) / kStripCols; }
 
 // stage: 0 = right-hand side only (out = rhs), 1..3 = SSP-RK3 stage with floor.
 // bc: 0 = reflecting walls (reference), 1 = transmissive (zero-gradient).
@@ -60,6 +60,44 @@ hipError_t launch_finalize_dt(const double *partial, int n, double cfl, double d
 // two-pass order of omp_solver.hpp:340-432 including the corner cells.
 hipError_t launch_ghost_fill(int bc, const GridDev &G, double *h, double *hu, double *hv,
                              hipStream_t stream);
+
+// ---- batched ensemble ---------------------------------------------------------------
+// Instances of uneven size share one slab per field; instance i's arrays start at element
+// offset[i] and have its own reference layout (ng = 2, stride = ny + 4).
+struct BatchInst {
+    int nx, ny, stride, pad;
+    long long offset;
+    double dx, dy, inv_dx, inv_dy;
+};
+
+// One entry of the tile work-list: instance, 60-column strip, first interior row.
+struct BatchTile {
+    int inst, strip, r0, pad;
+};
+
+struct BatchArgs {
+    StageArgs base;               // slab pointers (in_*, u0_*, B, out_*); tile_rows
+    const BatchInst *inst;
+    const BatchTile *tiles;
+    int ntiles;
+    const double *dt;             // per instance
+    unsigned long long *max_bits; // per instance, bits of a non-negative double (SPEED)
+    int *flags;                   // per instance non-finite flag
+    double g, half_g;
+    int split_ok;
+};
+
+// Stage launch over the work-list: `blocks` blocks of 4 waves; wave w takes tiles
+// w, w + W, w + 2W, ... (W = 4*blocks), i.e. persistent when blocks*4 < ntiles.
+hipError_t launch_batch_stage(int stage, int bc, bool speed, bool fast, const BatchArgs &B,
+                              int blocks, hipStream_t stream);
+// Max wave speed of each instance's current state (U = base.in_*) into max_bits.
+hipError_t launch_batch_speed(const BatchArgs &B, int blocks, hipStream_t stream);
+// Per instance: dt = compute_stable_dt expression of max_bits (then zeroed); history row
+// hist[*counter * count + i] when hist != null and *counter < hist_rows; counter += 1.
+hipError_t launch_batch_finalize(const BatchInst *inst, int count, unsigned long long *max_bits,
+                                 double cfl, double *dt, double *hist, long long *counter,
+                                 long long hist_rows, hipStream_t stream);
 
 hipError_t launch_set_scalar(double *dst, double v, hipStream_t stream);
 hipError_t launch_fill_int(int *dst, int v, int n, hipStream_t stream);

@@ -31,6 +31,9 @@ struct Cell {
     double h, hu, hv, b, eta;
 };
 
+// First column (lane 0, a halo column) of strip `strip` for ng = 2.
+__device__ __forceinline__ int strip_first_col(int strip) { return strip * kStripCols; }
+
 // Lane l receives lane l+1's value / lane l-1's value, through the DPP wavefront shifts
 // wave_shl:1 (0x130) and wave_shr:1 (0x138) of the GFX9 ISA: a VALU move, no LDS round
 // trip.  The lane without a source (63 resp. 0) keeps its own value; only lanes 1..62
@@ -64,11 +67,12 @@ __device__ __forceinline__ int mirror(int r, int lo, int hi, bool &ghost) {
 // The row offset is wave-uniform and the column is fixed per lane, so every access is a
 // scalar row base plus a constant 32-bit per-lane byte offset: the global_load/store
 // "saddr + voffset" form, with no per-row 64-bit address arithmetic on the vector ALUs.
-__device__ __forceinline__ const double *row_ptr(const double *base, int row_off) {
-    return base + __builtin_amdgcn_readfirstlane(row_off);
+// row_off = instance offset (0 for a single grid) + row * stride, wave-uniform.
+__device__ __forceinline__ const double *row_ptr(const double *base, long long row_off) {
+    return base + row_off;
 }
-__device__ __forceinline__ double *row_ptr(double *base, int row_off) {
-    return base + __builtin_amdgcn_readfirstlane(row_off);
+__device__ __forceinline__ double *row_ptr(double *base, long long row_off) {
+    return base + row_off;
 }
 __device__ __forceinline__ double ld(const double *row, unsigned byte_off) {
     return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(row) + byte_off);
@@ -78,7 +82,7 @@ __device__ __forceinline__ void st(double *row, unsigned byte_off, double v) {
 }
 
 template <int BC>
-__device__ __forceinline__ Cell load_cell(const StageArgs &A, int row_off, bool ghost_row,
+__device__ __forceinline__ Cell load_cell(const StageArgs &A, long long row_off, bool ghost_row,
                                           unsigned jc_bytes, bool ghost_col) {
     Cell c;
     c.h = ld(row_ptr(A.in_h, row_off), jc_bytes);
@@ -105,6 +109,7 @@ struct V3 {
 template <int STAGE, int BC, bool SPEED, bool FAST>
 struct March {
     const StageArgs &A;
+    long long base;  // element offset of this grid in the arrays (batched instances)
     int s, ng, nx, rmax, r1;
     unsigned j_bytes, jc_bytes;  // own column / source column, in bytes from the row start
     bool gcol, owner;
@@ -118,22 +123,25 @@ struct March {
     __device__ __forceinline__ Cell load(int r) {
         bool gr;
         const int rs = mirror<BC>(r, ng, ng + nx, gr);
-        const Cell c = load_cell<BC>(A, rs * s, gr, jc_bytes, gcol);
+        const Cell c = load_cell<BC>(A, base + __builtin_amdgcn_readfirstlane(rs * s), gr,
+                                     jc_bytes, gcol);
         split = split && __all(split_ok(c.h) && split_ok(c.hu) && split_ok(c.hv) && split_ok(c.b));
         return c;
     }
 
     __device__ __forceinline__ V3 load_u0(int r) const {
-        return V3{ld(row_ptr(A.u0_h, r * s), jc_bytes), ld(row_ptr(A.u0_hu, r * s), jc_bytes),
-                  ld(row_ptr(A.u0_hv, r * s), jc_bytes)};
+        const long long ro = base + __builtin_amdgcn_readfirstlane(r * s);
+        return V3{ld(row_ptr(A.u0_h, ro), jc_bytes), ld(row_ptr(A.u0_hu, ro), jc_bytes),
+                  ld(row_ptr(A.u0_hv, ro), jc_bytes)};
     }
 
     __device__ __forceinline__ void row(int r, const Cell &a, const Cell &b, const Cell &n,
                                         Cell &nn, const V3 &sa, V3 &sb, const V3 &d2, V3 &d3,
-                                        const V3 &fxp, V3 &fxn, const V3 &u0, V3 &u0n) {
-        // prefetch row r+3 and U^n at row r+1
+                                        const V3 &fxp, V3 &fxn) {
+        // prefetch row r+3; U^n of row r is loaded now and first used ~500 instructions on
         nn = load(r + 3 <= rmax ? r + 3 : rmax);
-        if (STAGE >= 2) u0n = load_u0(r + 1 < r1 ? r + 1 : r);
+        V3 u0{0.0, 0.0, 0.0};
+        if (STAGE >= 2) u0 = load_u0(r);
         // FAST has its own reciprocal arithmetic; only the exact mode has a split variant.
         if (!FAST && split)
             row_body<true>(r, a, b, n, sa, sb, d2, d3, fxp, fxn, u0);
@@ -176,7 +184,7 @@ struct March {
         const double rhv = divergence(inv_dx, inv_dy, fx.ft, fxp.v, fy.fn_m, fyp_hv);
 
         if (owner) {
-            const int ro = r * s;
+            const long long ro = base + __builtin_amdgcn_readfirstlane(r * s);
             if (STAGE == 0) {
                 st(row_ptr(A.out_h, ro), j_bytes, rh);
                 st(row_ptr(A.out_hu, ro), j_bytes, rhu);
@@ -211,6 +219,61 @@ struct March {
 #define SWE_STAGE_WAVES_PER_EU 3
 #endif
 
+// One wave's tile: strip `strip` (60 output columns) of rows [r0, min(r0 + tile_rows,
+// ng + nx)).  Accumulates the max wave speed of the written cells (SPEED) and a non-finite
+// flag into the caller's per-lane values.
+template <int STAGE, int BC, bool SPEED, bool FAST>
+__device__ __forceinline__ void run_tile(const GridDev &G, const StageArgs &A, long long base,
+                                         double dt, int strip, int r0, int lane, double &speed,
+                                         int &bad) {
+    const int ng = G.ng, nx = G.nx, ny = G.ny;
+    const int j = ng - 2 + strip * kStripCols + lane;
+    bool gcol;
+    const int jc = mirror<BC>(j < ng + ny + 2 ? j : ng + ny + 1, ng, ng + ny, gcol);
+    const int r1 = (r0 + A.tile_rows < ng + nx) ? r0 + A.tile_rows : ng + nx;
+
+    March<STAGE, BC, SPEED, FAST> m{A, base, G.stride, ng, nx, ng + nx + 1, r1,
+                                    (unsigned)j * 8u, (unsigned)jc * 8u, gcol,
+                                    lane >= 2 && lane < kWave - 2 && j < ng + ny,
+                                    G.half_g, G.g, G.inv_dx, G.inv_dy, dt, speed, bad,
+                                    G.split_ok != 0};
+
+    // ---- prologue: rows r0-2 .. r0+2 and the plus-side flux of face r0-1/2
+    const Cell P2 = m.load(r0 - 2);
+    const Cell P1 = m.load(r0 - 1);
+    Cell C0 = m.load(r0), C1 = m.load(r0 + 1), C2 = m.load(r0 + 2 <= m.rmax ? r0 + 2 : m.rmax);
+    Cell C3;
+
+    // x differences d_{r0-3/2}, d_{r0-1/2}, d_{r0+1/2}
+    const V3 dm{P1.eta - P2.eta, P1.hu - P2.hu, P1.hv - P2.hv};
+    const V3 d0{C0.eta - P1.eta, C0.hu - P1.hu, C0.hv - P1.hv};
+    V3 D0{C1.eta - C0.eta, C1.hu - C0.hu, C1.hv - C0.hv}, D1;
+    // minus-side reconstruction of row r0-1 at face r0-1/2, slope of row r0
+    const V3 sp{0.5 * minmod(dm.e, d0.e), 0.5 * minmod(dm.u, d0.u), 0.5 * minmod(dm.v, d0.v)};
+    V3 S0{0.5 * minmod(d0.e, D0.e), 0.5 * minmod(d0.u, D0.u), 0.5 * minmod(d0.v, D0.v)}, S1;
+    const FaceFlux f0 =
+        (!FAST && m.split) ? face_flux<FAST, true>(m.half_g, m.g, P1.eta + sp.e, P1.hu + sp.u,
+                                                   P1.hv + sp.v, P1.b, C0.eta - S0.e,
+                                                   C0.hu - S0.u, C0.hv - S0.v, C0.b)
+                           : face_flux<FAST, false>(m.half_g, m.g, P1.eta + sp.e, P1.hu + sp.u,
+                                                    P1.hv + sp.v, P1.b, C0.eta - S0.e,
+                                                    C0.hu - S0.u, C0.hv - S0.v, C0.b);
+    V3 F0{f0.fh, f0.fn_p, f0.ft}, F1;
+
+    // ---- march, unrolled 4x so the register window rotates by renaming
+    for (int r = r0; r < r1; r += 4) {
+        m.row(r, C0, C1, C2, C3, S0, S1, D0, D1, F0, F1);
+        if (r + 1 >= r1) break;
+        m.row(r + 1, C1, C2, C3, C0, S1, S0, D1, D0, F1, F0);
+        if (r + 2 >= r1) break;
+        m.row(r + 2, C2, C3, C0, C1, S0, S1, D0, D1, F0, F1);
+        if (r + 3 >= r1) break;
+        m.row(r + 3, C3, C0, C1, C2, S1, S0, D1, D0, F1, F0);
+    }
+    speed = m.speed;
+    bad = m.bad;
+}
+
 template <int STAGE, int BC, bool SPEED, bool FAST>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SWE_STAGE_WAVES_PER_EU)))
 void stage_kernel(const GridDev G, const StageArgs A) {
@@ -221,57 +284,9 @@ void stage_kernel(const GridDev G, const StageArgs A) {
     double speed = 0.0;
     int bad = 0;
 
-    if (strip < A.nstrips) {
-        const int ng = G.ng, nx = G.nx, ny = G.ny;
-        const int j = ng - 2 + strip * kStripCols + lane;
-        bool gcol;
-        const int jc = mirror<BC>(j < ng + ny + 2 ? j : ng + ny + 1, ng, ng + ny, gcol);
-        const int r0 = ng + blockIdx.y * A.tile_rows;
-        const int r1 = (r0 + A.tile_rows < ng + nx) ? r0 + A.tile_rows : ng + nx;
-
-        March<STAGE, BC, SPEED, FAST> m{A, G.stride, ng, nx, ng + nx + 1, r1,
-                                        (unsigned)j * 8u, (unsigned)jc * 8u, gcol,
-                                        lane >= 2 && lane < kWave - 2 && j < ng + ny,
-                                        G.half_g, G.g, G.inv_dx, G.inv_dy,
-                                        STAGE != 0 ? *A.dt : 0.0, 0.0, 0, G.split_ok != 0};
-
-        // ---- prologue: rows r0-2 .. r0+2 and the plus-side flux of face r0-1/2
-        const Cell P2 = m.load(r0 - 2);
-        const Cell P1 = m.load(r0 - 1);
-        Cell C0 = m.load(r0), C1 = m.load(r0 + 1), C2 = m.load(r0 + 2 <= m.rmax ? r0 + 2 : m.rmax);
-        Cell C3;
-        V3 U0{0.0, 0.0, 0.0}, U1{0.0, 0.0, 0.0};
-        if (STAGE >= 2) U0 = m.load_u0(r0);
-
-        // x differences d_{r0-3/2}, d_{r0-1/2}, d_{r0+1/2}
-        const V3 dm{P1.eta - P2.eta, P1.hu - P2.hu, P1.hv - P2.hv};
-        const V3 d0{C0.eta - P1.eta, C0.hu - P1.hu, C0.hv - P1.hv};
-        V3 D0{C1.eta - C0.eta, C1.hu - C0.hu, C1.hv - C0.hv}, D1;
-        // minus-side reconstruction of row r0-1 at face r0-1/2, slope of row r0
-        const V3 sp{0.5 * minmod(dm.e, d0.e), 0.5 * minmod(dm.u, d0.u), 0.5 * minmod(dm.v, d0.v)};
-        V3 S0{0.5 * minmod(d0.e, D0.e), 0.5 * minmod(d0.u, D0.u), 0.5 * minmod(d0.v, D0.v)}, S1;
-        const FaceFlux f0 =
-            (!FAST && m.split) ? face_flux<FAST, true>(m.half_g, m.g, P1.eta + sp.e, P1.hu + sp.u,
-                                            P1.hv + sp.v, P1.b, C0.eta - S0.e, C0.hu - S0.u,
-                                            C0.hv - S0.v, C0.b)
-                    : face_flux<FAST, false>(m.half_g, m.g, P1.eta + sp.e, P1.hu + sp.u,
-                                             P1.hv + sp.v, P1.b, C0.eta - S0.e, C0.hu - S0.u,
-                                             C0.hv - S0.v, C0.b);
-        V3 F0{f0.fh, f0.fn_p, f0.ft}, F1;
-
-        // ---- march, unrolled 4x so the register window rotates by renaming
-        for (int r = r0; r < r1; r += 4) {
-            m.row(r, C0, C1, C2, C3, S0, S1, D0, D1, F0, F1, U0, U1);
-            if (r + 1 >= r1) break;
-            m.row(r + 1, C1, C2, C3, C0, S1, S0, D1, D0, F1, F0, U1, U0);
-            if (r + 2 >= r1) break;
-            m.row(r + 2, C2, C3, C0, C1, S0, S1, D0, D1, F0, F1, U0, U1);
-            if (r + 3 >= r1) break;
-            m.row(r + 3, C3, C0, C1, C2, S1, S0, D1, D0, F1, F0, U1, U0);
-        }
-        speed = m.speed;
-        bad = m.bad;
-    }
+    if (strip < A.nstrips)
+        run_tile<STAGE, BC, SPEED, FAST>(G, A, 0, STAGE != 0 ? *A.dt : 0.0, strip,
+                                         G.ng + (int)blockIdx.y * A.tile_rows, lane, speed, bad);
 
     if (STAGE != 0) {
         if (__any(bad)) {
@@ -288,6 +303,108 @@ void stage_kernel(const GridDev G, const StageArgs A) {
             for (int w = 1; w < kBlockWaves; ++w) mx = smax(mx, red[w]);
             A.partial[blockIdx.y * gridDim.x + blockIdx.x] = mx;
         }
+    }
+}
+
+// ---- batched ensemble ----------------------------------------------------------------
+
+__device__ __forceinline__ GridDev inst_grid(const BatchArgs &B, const BatchInst &I) {
+    return GridDev{I.nx, I.ny, 2, I.stride, I.dx, I.dy, I.inv_dx, I.inv_dy, B.g, B.half_g,
+                   B.split_ok};
+}
+
+__device__ __forceinline__ double wave_max(double v) {
+    for (int off = kWave / 2; off > 0; off >>= 1) v = smax(v, __shfl_xor(v, off));
+    return v;
+}
+
+// Persistent tile loop.  Every wave walks the work-list with stride W; the tile index is
+// wave-uniform, so the tile and instance records are scalar loads.  Per-instance max wave
+// speed goes to max_bits[inst] with a 64-bit atomic max on the bit pattern (exact for
+// non-negative doubles, order-independent); non-finite flags per instance.
+template <int STAGE, int BC, bool SPEED, bool FAST>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SWE_STAGE_WAVES_PER_EU)))
+void batch_stage_kernel(const BatchArgs B) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int gw = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave));
+    const int nw = (int)(gridDim.x * blockDim.x / kWave);
+    for (int t = gw; t < B.ntiles; t += nw) {
+        const BatchTile T = B.tiles[t];
+        const BatchInst I = B.inst[T.inst];
+        const GridDev G = inst_grid(B, I);
+        double speed = 0.0;
+        int bad = 0;
+        run_tile<STAGE, BC, SPEED, FAST>(G, B.base, I.offset, STAGE != 0 ? B.dt[T.inst] : 0.0,
+                                         T.strip, T.r0, lane, speed, bad);
+        if (SPEED) {
+            speed = wave_max(speed);
+            if (lane == 0) atomicMax(&B.max_bits[T.inst], (unsigned long long)__double_as_longlong(speed));
+        }
+        if (STAGE != 0 && __any(bad)) {
+            if (lane == 0) atomicOr(&B.flags[T.inst], 1);
+        }
+    }
+}
+
+// Max wave speed of U over each tile's cells (the owner lanes of its rows).
+__global__ __launch_bounds__(256) void batch_speed_kernel(const BatchArgs B) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int gw = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave));
+    const int nw = (int)(gridDim.x * blockDim.x / kWave);
+    for (int t = gw; t < B.ntiles; t += nw) {
+        const BatchTile T = B.tiles[t];
+        const BatchInst I = B.inst[T.inst];
+        const int j = strip_first_col(T.strip) + lane;
+        const bool owner = lane >= 2 && lane < kWave - 2 && j < 2 + I.ny;
+        const int r1 = (T.r0 + B.base.tile_rows < 2 + I.nx) ? T.r0 + B.base.tile_rows : 2 + I.nx;
+        double m = 0.0;
+        if (owner) {
+            const double *h = B.base.in_h + I.offset, *hu = B.base.in_hu + I.offset,
+                         *hv = B.base.in_hv + I.offset;
+            for (int r = T.r0; r < r1; ++r) {
+                const long long k = (long long)r * I.stride + j;
+                m = cell_speed(B.g, h[k], hu[k], hv[k], m);
+            }
+        }
+        m = wave_max(m);
+        if (lane == 0) atomicMax(&B.max_bits[T.inst], (unsigned long long)__double_as_longlong(m));
+    }
+}
+
+__global__ __launch_bounds__(1024) void batch_finalize_kernel(const BatchInst *inst, int count,
+                                                             unsigned long long *max_bits,
+                                                             double cfl, double *dt,
+                                                             double *hist, long long *counter,
+                                                             long long hist_rows) {
+    const long long c = counter ? *counter : 0;
+    for (int i = threadIdx.x; i < count; i += blockDim.x) {
+        const double ms = __longlong_as_double((long long)max_bits[i]);
+        const double d = dt_from_speed(ms, cfl, inst[i].dx, inst[i].dy);
+        dt[i] = d;
+        max_bits[i] = 0ull;
+        if (hist != nullptr && c < hist_rows) hist[c * count + i] = d;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0 && counter != nullptr) *counter = c + 1;
+}
+
+template <int STAGE, int BC, bool SPEED, bool FAST>
+hipError_t launch_batch_one(const BatchArgs &B, int blocks, hipStream_t stream) {
+    hipLaunchKernelGGL((batch_stage_kernel<STAGE, BC, SPEED, FAST>), dim3(blocks),
+                       dim3(kWave * kBlockWaves), 0, stream, B);
+    return hipGetLastError();
+}
+
+template <int BC, bool FAST>
+hipError_t launch_batch_bc(int stage, bool speed, const BatchArgs &B, int blocks,
+                           hipStream_t stream) {
+    switch (stage) {
+        case 1: return launch_batch_one<1, BC, false, FAST>(B, blocks, stream);
+        case 2: return launch_batch_one<2, BC, false, FAST>(B, blocks, stream);
+        case 3:
+            return speed ? launch_batch_one<3, BC, true, FAST>(B, blocks, stream)
+                         : launch_batch_one<3, BC, false, FAST>(B, blocks, stream);
+        default: return hipErrorInvalidValue;
     }
 }
 
@@ -430,6 +547,30 @@ hipError_t launch_stage(int stage, int bc, bool speed, bool fast, const GridDev 
                        : launch_bc<1, true>(stage, speed, G, A, stream, grid);
     return bc == 0 ? launch_bc<0, false>(stage, speed, G, A, stream, grid)
                    : launch_bc<1, false>(stage, speed, G, A, stream, grid);
+}
+
+hipError_t launch_batch_stage(int stage, int bc, bool speed, bool fast, const BatchArgs &B,
+                              int blocks, hipStream_t stream) {
+    if (blocks < 1 || B.ntiles < 1 || B.base.tile_rows < 1) return hipErrorInvalidValue;
+    if (fast)
+        return bc == 0 ? launch_batch_bc<0, true>(stage, speed, B, blocks, stream)
+                       : launch_batch_bc<1, true>(stage, speed, B, blocks, stream);
+    return bc == 0 ? launch_batch_bc<0, false>(stage, speed, B, blocks, stream)
+                   : launch_batch_bc<1, false>(stage, speed, B, blocks, stream);
+}
+
+hipError_t launch_batch_speed(const BatchArgs &B, int blocks, hipStream_t stream) {
+    hipLaunchKernelGGL(batch_speed_kernel, dim3(blocks), dim3(kWave * kBlockWaves), 0, stream,
+                       B);
+    return hipGetLastError();
+}
+
+hipError_t launch_batch_finalize(const BatchInst *inst, int count, unsigned long long *max_bits,
+                                 double cfl, double *dt, double *hist, long long *counter,
+                                 long long hist_rows, hipStream_t stream) {
+    hipLaunchKernelGGL(batch_finalize_kernel, dim3(1), dim3(1024), 0, stream, inst, count,
+                       max_bits, cfl, dt, hist, counter, hist_rows);
+    return hipGetLastError();
 }
 
 hipError_t launch_speed_reduce(const GridDev &G, const double *h, const double *hu,

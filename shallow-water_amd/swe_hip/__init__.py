@@ -74,6 +74,24 @@ _SIGS = {
     "swe_hip_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "swe_hip_time_steps": (C.c_int, [C.c_void_p, C.c_double, C.c_int64, _dp, _dp]),
     "swe_hip_sync": (C.c_int, [C.c_void_p]),
+    "swe_hip_batch_create": (C.c_int, [C.POINTER(GridDesc), C.c_int32, C.c_int,
+                                       C.POINTER(C.c_void_p)]),
+    "swe_hip_batch_destroy": (None, [C.c_void_p]),
+    "swe_hip_batch_last_error": (C.c_char_p, [C.c_void_p]),
+    "swe_hip_batch_count": (C.c_int, [C.c_void_p]),
+    "swe_hip_batch_upload": (C.c_int, [C.c_void_p, C.c_int32, _dp, _dp, _dp, _dp]),
+    "swe_hip_batch_download": (C.c_int, [C.c_void_p, C.c_int32, _dp, _dp, _dp]),
+    "swe_hip_batch_set_dt": (C.c_int, [C.c_void_p, _dp]),
+    "swe_hip_batch_stable_dt": (C.c_int, [C.c_void_p, C.c_double, _dp]),
+    "swe_hip_batch_step": (C.c_int, [C.c_void_p, C.c_int64]),
+    "swe_hip_batch_step_adaptive": (C.c_int, [C.c_void_p, C.c_double, C.c_int64, _dp]),
+    "swe_hip_batch_nonfinite": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32)]),
+    "swe_hip_batch_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "swe_hip_batch_get_launch": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "swe_hip_batch_autotune": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
+    "swe_hip_batch_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
+    "swe_hip_batch_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
+    "swe_hip_batch_sync": (C.c_int, [C.c_void_p]),
 }
 
 _lib = None
@@ -225,3 +243,101 @@ class HipSolver:
 
     def sync(self):
         self._check(lib().swe_hip_sync(self._ctx))
+
+
+class HipBatch:
+    """An ensemble of independent grids advanced together (swe_hip_batch_* in the C ABI).
+
+    cases: sequence of workload dicts (nx, ny, dx, dy, g, bc) -- all with ng = 2 and the
+    same g and bc."""
+
+    def __init__(self, cases, device=0):
+        self.count = len(cases)
+        descs = (GridDesc * self.count)()
+        self.shapes = []
+        for i, c in enumerate(cases):
+            descs[i] = GridDesc(int(c["nx"]), int(c["ny"]), int(c.get("ng", 2)), float(c["dx"]),
+                                float(c["dy"]), float(c["g"]), int(c["bc"]))
+            self.shapes.append((int(c["nx"]) + 4, int(c["ny"]) + 4))
+        self._b = C.c_void_p()
+        self._check(lib().swe_hip_batch_create(descs, self.count, int(device), C.byref(self._b)))
+
+    def _check(self, rc):
+        if rc != SWE_OK:
+            L = lib()
+            detail = (L.swe_hip_batch_last_error(self._b) or b"").decode() if self._b else ""
+            raise SweError(rc, L.swe_hip_error_string(rc).decode() + (f": {detail}" if detail else ""))
+
+    def close(self):
+        if self._b:
+            lib().swe_hip_batch_destroy(self._b)
+            self._b = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def upload(self, i, h, hu, hv, B):
+        arrs = [np.ascontiguousarray(a, dtype=np.float64) for a in (h, hu, hv, B)]
+        for a in arrs:
+            if a.shape != self.shapes[i]:
+                raise ValueError(f"instance {i}: expected {self.shapes[i]}, got {a.shape}")
+        self._check(lib().swe_hip_batch_upload(self._b, int(i), *(_p(a) for a in arrs)))
+
+    def download(self, i):
+        out = [np.empty(self.shapes[i]) for _ in range(3)]
+        self._check(lib().swe_hip_batch_download(self._b, int(i), *(_p(o) for o in out)))
+        return out
+
+    def set_dt(self, dt):
+        dt = np.ascontiguousarray(dt, dtype=np.float64)
+        assert dt.shape == (self.count,)
+        self._check(lib().swe_hip_batch_set_dt(self._b, _p(dt)))
+
+    def stable_dt(self, cfl):
+        out = np.zeros(self.count)
+        self._check(lib().swe_hip_batch_stable_dt(self._b, float(cfl), _p(out)))
+        return out
+
+    def step(self, nsteps=1):
+        self._check(lib().swe_hip_batch_step(self._b, int(nsteps)))
+
+    def step_adaptive(self, cfl, nsteps=1):
+        hist = np.zeros((max(int(nsteps), 1), self.count))
+        self._check(lib().swe_hip_batch_step_adaptive(self._b, float(cfl), int(nsteps), _p(hist)))
+        return hist[:nsteps]
+
+    def nonfinite(self):
+        f = (C.c_int32 * self.count)()
+        self._check(lib().swe_hip_batch_nonfinite(self._b, f))
+        return np.array(f[:], dtype=np.int32)
+
+    def set_launch(self, tile_rows, waves_per_cu=0):
+        self._check(lib().swe_hip_batch_set_launch(self._b, int(tile_rows), int(waves_per_cu)))
+
+    def get_launch(self):
+        t, w = C.c_int(), C.c_int()
+        self._check(lib().swe_hip_batch_get_launch(self._b, C.byref(t), C.byref(w)))
+        return t.value, w.value
+
+    def autotune(self):
+        t, w = C.c_int(), C.c_int()
+        self._check(lib().swe_hip_batch_autotune(self._b, C.byref(t), C.byref(w)))
+        return t.value, w.value
+
+    def set_precision(self, mode):
+        self._check(lib().swe_hip_batch_set_precision(self._b, int(mode)))
+
+    def set_graph(self, enabled: bool):
+        self._check(lib().swe_hip_batch_set_graph(self._b, int(bool(enabled))))
+
+    def sync(self):
+        self._check(lib().swe_hip_batch_sync(self._b))
