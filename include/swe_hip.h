@@ -184,6 +184,62 @@ int swe_hip_batch_set_precision(swe_hip_batch *batch, int mode);
 int swe_hip_batch_set_graph(swe_hip_batch *batch, int enabled);
 int swe_hip_batch_sync(swe_hip_batch *batch);
 
+/* ---- discrete adjoint -------------------------------------------------------------------
+ * The reference has no differentiation (its solvers only step forward); these entry points
+ * are this backend's own.  They reverse the reference's SSP-RK3 step
+ * (omp_solver.hpp:250-338) stage by stage with dt frozen from the forward pass, under the
+ * one-sided derivative conventions documented in csrc/swe_adjoint.hip, and recompute the
+ * forward from binomial checkpoints (optimal schedule for the given number of stored
+ * states) held in device memory.
+ *
+ * Usage: create on a context whose state holds U^0 (after swe_hip_upload), forward (runs
+ * nsteps, leaves U^N in the context), backward once with dL/dU^N.  A backward consumes the
+ * checkpoints; call forward again before another backward.  The adjoint object uses the
+ * context's stream, B and stage buffers: destroy it before the context.  ng must be 2. */
+typedef struct swe_hip_adj swe_hip_adj;
+
+typedef enum {
+    SWE_DT_FIXED = 0,    /* one dt for every step */
+    SWE_DT_ADAPTIVE = 1, /* dt from the CFL number at each step's start state (recorded) */
+    SWE_DT_GIVEN = 2     /* a caller-given dt per step */
+} swe_dt_mode;
+
+/* nsnap = number of stored states, U^0 included (>= 1); 0 picks min(nsteps, what half of
+ * the free device memory holds). */
+int swe_hip_adj_create(swe_hip_ctx *ctx, int64_t nsteps, int32_t nsnap, swe_hip_adj **out);
+void swe_hip_adj_destroy(swe_hip_adj *adj);
+const char *swe_hip_adj_last_error(const swe_hip_adj *adj);
+
+/* Forward pass from the context's current state.  dt_mode FIXED: value = dt; ADAPTIVE:
+ * value = CFL number; GIVEN: dt_hist holds nsteps dts (input).  For FIXED and ADAPTIVE,
+ * dt_hist (may be NULL) receives the nsteps dts used.  Leaves U^N in the context. */
+int swe_hip_adj_forward(swe_hip_adj *adj, int32_t dt_mode, double value, double *dt_hist);
+
+/* Backward pass: lam_* = dL/dU^N (n_total host arrays; ghost entries ignored) ->
+ * grad_* = dL/dU^0 and grad_b = dL/dB (interior entries; ghost entries 0).  grad_b may be
+ * NULL. */
+int swe_hip_adj_backward(swe_hip_adj *adj, const double *lam_h, const double *lam_hu,
+                         const double *lam_hv, double *grad_h, double *grad_hu,
+                         double *grad_hv, double *grad_b);
+/* Same with device pointers (on the context's device). */
+int swe_hip_adj_backward_device(swe_hip_adj *adj, const double *lam_h, const double *lam_hu,
+                                const double *lam_hv, double *grad_h, double *grad_hu,
+                                double *grad_hv, double *grad_b);
+
+/* Rows per wave of the adjoint stage kernel (>= 1). */
+int swe_hip_adj_set_tile_rows(swe_hip_adj *adj, int tile_rows);
+
+/* The checkpoint schedule for (nsteps, nsnap), computed on the host (no device needed):
+ * total forward steps advanced (the first sweep included), the most times any one step is
+ * advanced, and the number of schedule actions. */
+int swe_hip_adj_schedule(int64_t nsteps, int32_t nsnap, int64_t *advances, int32_t *max_reps,
+                         int64_t *actions);
+
+/* Wall-clock milliseconds of the last forward and backward calls, and the number of stored
+ * states in use. */
+int swe_hip_adj_info(const swe_hip_adj *adj, int32_t *nsnap, double *forward_ms,
+                     double *backward_ms);
+
 #ifdef __cplusplus
 }
 #endif

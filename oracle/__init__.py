@@ -9,6 +9,9 @@ Two libraries:
 * ``liboracle.so`` -- this build's own C restatement (oracle/swe_oracle.c) of the
   reference's ``include/backend/openmp/omp_kernels.hpp`` plus the ``rk3_step`` driver
   of ``include/backend/openmp/omp_solver.hpp:250-338``.  g is a run-time parameter.
+* ``libswe_tangent.so`` -- a forward-mode (dual-number) tangent of the reference's serial
+  optimized solver (oracle/swe_tangent.cpp; include/backend/serial/HPC/), the checker of
+  the GPU discrete adjoint.  Its primal is bitwise the oracle's fixed-dt run.
 * ``_ref/libswe_ref.so`` -- the reference's own header-only kernels compiled where they
   lie (oracle/ref_driver.cpp + ``make -C oracle ref``).  Present only where the
   reference tree was available at build time (the .so itself travels with the tree).
@@ -27,6 +30,7 @@ import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
+TANGENT_SO = os.path.join(HERE, "libswe_tangent.so")
 REF_SO = os.path.join(HERE, "_ref", "libswe_ref.so")
 
 G_REF = 9.80665  # include/constants.hpp:4
@@ -46,7 +50,8 @@ class OracleGrid(C.Structure):
 
 def build(ref: bool | None = None) -> None:
     """Compile liboracle.so (and the reference harness when the reference tree exists)."""
-    subprocess.run(["make", "-s", "-C", HERE, "-B", "liboracle.so"], check=True)
+    subprocess.run(["make", "-s", "-C", HERE, "-B", "liboracle.so", "libswe_tangent.so"],
+                   check=True)
     ref_root = os.environ.get("SWE_REFERENCE_ROOT", "/root/reference")
     if ref is None:
         ref = os.path.isdir(os.path.join(ref_root, "include", "backend", "openmp"))
@@ -210,3 +215,38 @@ def ref_run(gr: OracleGrid, h, hu, hv, B, nsteps, dt=None, cfl=None):
     rc = ref().ref_run(gr.nx, gr.ny, gr.ng, gr.dx, gr.dy, _p(h), _p(hu), _p(hv), _p(B), mode, val,
                        nsteps, _p(hist))
     return rc, hist[:nsteps]
+
+
+_tan = None
+
+
+def tangent_lib():
+    global _tan
+    if _tan is None:
+        if not os.path.exists(TANGENT_SO):
+            subprocess.run(["make", "-s", "-C", HERE, "libswe_tangent.so"], check=True)
+        L = C.CDLL(TANGENT_SO)
+        L.tangent_primal.restype = C.c_int
+        L.tangent_primal.argtypes = ([C.c_int] * 3 + [C.c_double] * 3 + [C.c_int] + [_dp] * 5 +
+                                     [C.c_long])
+        L.tangent_run.restype = C.c_int
+        L.tangent_run.argtypes = ([C.c_int] * 3 + [C.c_double] * 3 + [C.c_int] + [_dp] * 9 +
+                                  [C.c_long])
+        _tan = L
+    return _tan
+
+
+def tangent_primal(gr, h, hu, hv, B, dts):
+    """Fixed-dt-sequence run of the tangent's primal (T = double); arrays in place."""
+    dts = np.ascontiguousarray(dts, dtype=np.float64)
+    return tangent_lib().tangent_primal(gr.nx, gr.ny, gr.ng, gr.dx, gr.dy, gr.g, gr.bc_kind,
+                                        _p(h), _p(hu), _p(hv), _p(B), _p(dts), len(dts))
+
+
+def tangent_run(gr, h, hu, hv, B, dh, dhu, dhv, dB, dts):
+    """Directional derivative of the final state along (dh, dhu, dhv, dB), dt frozen.
+    h/hu/hv become the final state, dh/dhu/dhv its tangent.  Returns 1 on non-finite."""
+    dts = np.ascontiguousarray(dts, dtype=np.float64)
+    return tangent_lib().tangent_run(gr.nx, gr.ny, gr.ng, gr.dx, gr.dy, gr.g, gr.bc_kind,
+                                     _p(h), _p(hu), _p(hv), _p(B), _p(dh), _p(dhu), _p(dhv),
+                                     _p(dB), _p(dts), len(dts))

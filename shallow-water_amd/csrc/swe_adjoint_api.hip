@@ -1,0 +1,459 @@
+// swe_adjoint_api.hip -- discrete adjoint driver: binomial checkpointing over the fused
+// stage kernels, and the adjoint entry points of include/swe_hip.h.
+//
+// Checkpoint schedule.  Reversing n steps whose start state sits in a stored slot, with f
+// more free slots (s = f + 1 stored states in all), costs at least r*n - C(s+r, r-1)
+// single-step advances, r the least integer with C(s+r, s) >= n (Griewank's binomial bound,
+// A. Griewank and A. Walther, "Algorithm 799: revolve", ACM TOMS 26(1), 2000).  The
+// recursion below attains it: advance k steps, store that state, reverse the right part
+// with f-1 free slots, then the left part with f; any k in
+//     [max(C(s+r-2, s), n - C(s-1+r, s-1)), min(C(s+r-1, s), n - C(s-1+r-1, s-1))]
+// is optimal, and the lower end is taken.  With f = 0 every step is reached by advancing
+// from the stored start.  Slots are used last-in first-out.
+//
+// Per reversed step i (state U^i = "base", dt_i frozen): recompute U1 and U2 with the
+// forward stage kernels, then run the adjoint stage kernels 3, 2, 1.  The floor masks of
+// the stage outputs travel between the adjoint kernels (each writes the mask of its own
+// input), so no stage output is ever recomputed for them.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/swe_hip.h"
+#include "swe_ctx.h"
+#include "swe_internal.h"
+
+namespace {
+
+enum { kAdvance = 0, kAdjoint = 1 };
+
+// kAdvance: state a (in slot src, -1 = work buffer) -> state b (into slot dst, -1 = work).
+// kAdjoint: reverse step a whose start state is in slot src (-1 = work buffer).
+struct Action {
+    int kind;
+    int64_t a, b;
+    int src, dst;
+};
+
+// C(s + r, r) saturated above `cap`.
+int64_t beta(int64_t s, int64_t r, int64_t cap) {
+    if (r < 0 || s < 0) return 0;
+    double b = 1.0;
+    for (int64_t i = 1; i <= r; ++i) {
+        b = b * (double)(s + i) / (double)i;
+        if (b > (double)cap) return cap + 1;
+    }
+    return (int64_t)(b + 0.5);
+}
+
+int64_t repetitions(int64_t n, int64_t s) {
+    int64_t r = 0;
+    while (beta(s, r, n) < n) ++r;
+    return r;
+}
+
+std::vector<Action> make_schedule(int64_t nsteps, int32_t nsnap) {
+    struct Task {
+        int64_t a, b;
+        int f, slot;
+    };
+    std::vector<Action> out;
+    std::vector<Task> stack{{0, nsteps, nsnap - 1, 0}};
+    while (!stack.empty()) {
+        const Task t = stack.back();
+        stack.pop_back();
+        const int64_t n = t.b - t.a;
+        if (n == 1) {
+            out.push_back({kAdjoint, t.a, t.b, t.slot, -1});
+            continue;
+        }
+        if (t.f == 0) {
+            for (int64_t i = t.b - 1; i >= t.a; --i) {
+                if (i > t.a) out.push_back({kAdvance, t.a, i, t.slot, -1});
+                out.push_back({kAdjoint, i, i + 1, i > t.a ? -1 : t.slot, -1});
+            }
+            continue;
+        }
+        const int64_t s = t.f + 1;
+        const int64_t r = repetitions(n, s);
+        const int64_t k = std::max<int64_t>({1, beta(s, r - 2, n), n - beta(s - 1, r, n)});
+        const int64_t m = t.a + k;
+        const int slot_m = nsnap - t.f;
+        out.push_back({kAdvance, t.a, m, t.slot, slot_m});
+        stack.push_back({t.a, m, t.f, t.slot});      // reversed after the right part
+        stack.push_back({m, t.b, t.f - 1, slot_m});  // next
+    }
+    return out;
+}
+
+}  // namespace
+
+struct swe_hip_adj {
+    swe_hip_ctx *ctx = nullptr;
+    int64_t nsteps = 0;
+    int32_t nsnap = 0;
+    size_t n_total = 0;
+    double *slots = nullptr;  // nsnap states of 3 arrays
+    double *W[3] = {nullptr, nullptr, nullptr};
+    double *lam[4][3] = {};   // step adjoint, dL/dU2, dL/dU1, U^n accumulator
+    double *lam_b = nullptr;
+    unsigned char *mask[2] = {nullptr, nullptr};
+    double *dts = nullptr;
+    double *partial = nullptr;  // 1024 per-block maxima (adaptive dt)
+    long long *counter = nullptr;
+    std::vector<Action> acts;
+    size_t first_adj = 0;
+    int tile_rows = 16;
+    int state = 0;  // 0 created, 1 forward done (checkpoints valid), 2 consumed
+    int parity = 0;
+    double fwd_ms = 0.0, bwd_ms = 0.0;
+    std::string err;
+};
+
+namespace {
+
+int afail(swe_hip_adj *a, int code, const std::string &msg) {
+    if (a) a->err = msg;
+    return code;
+}
+
+#define ADJ_CHECK(adj, call)                                                                \
+    do {                                                                                    \
+        const hipError_t e__ = (call);                                                      \
+        if (e__ != hipSuccess)                                                              \
+            return afail(adj, SWE_ERR_HIP, std::string(#call) + ": " + hipGetErrorString(e__)); \
+    } while (0)
+
+double *slot_ptr(swe_hip_adj *a, int slot, int comp) {
+    return a->slots + ((size_t)slot * 3 + comp) * a->n_total;
+}
+
+struct State {
+    double *p[3];
+};
+
+State state_of(swe_hip_adj *a, int slot) {
+    if (slot < 0) return State{{a->W[0], a->W[1], a->W[2]}};
+    return State{{slot_ptr(a, slot, 0), slot_ptr(a, slot, 1), slot_ptr(a, slot, 2)}};
+}
+
+swe::StageArgs fwd_args(swe_hip_adj *a, const State &in, const State &u0, const State &out,
+                        const double *dt) {
+    swe_hip_ctx *c = a->ctx;
+    swe::StageArgs s{};
+    s.in_h = in.p[0];
+    s.in_hu = in.p[1];
+    s.in_hv = in.p[2];
+    s.u0_h = u0.p[0];
+    s.u0_hu = u0.p[1];
+    s.u0_hv = u0.p[2];
+    s.B = c->B;
+    s.out_h = out.p[0];
+    s.out_hu = out.p[1];
+    s.out_hv = out.p[2];
+    s.dt = dt;
+    s.partial = nullptr;
+    s.flag = c->flag;
+    s.tile_rows = c->tile_rows;
+    s.nstrips = swe::strips_for(c->G.ny);
+    return s;
+}
+
+// Stages 1 and 2 of step p from `base` into the context's U1, U2.
+int recompute_u1_u2(swe_hip_adj *a, const State &base, int64_t p) {
+    swe_hip_ctx *c = a->ctx;
+    const State u1{{c->U1[0], c->U1[1], c->U1[2]}}, u2{{c->U2[0], c->U2[1], c->U2[2]}};
+    ADJ_CHECK(a, swe::launch_stage(1, c->bc, false, false, c->G,
+                                   fwd_args(a, base, base, u1, a->dts + p), c->stream, nullptr));
+    ADJ_CHECK(a, swe::launch_stage(2, c->bc, false, false, c->G,
+                                   fwd_args(a, u1, base, u2, a->dts + p), c->stream, nullptr));
+    return SWE_OK;
+}
+
+// One forward step p: in -> out (out may alias in).  With cfl > 0, dt_p is first computed
+// from `in` (compute_stable_dt) and recorded in dts[p].
+int forward_step(swe_hip_adj *a, const State &in, const State &out, int64_t p, double cfl) {
+    swe_hip_ctx *c = a->ctx;
+    if (cfl > 0.0) {
+        ADJ_CHECK(a, swe::launch_speed_reduce(c->G, in.p[0], in.p[1], in.p[2], a->partial, 1024,
+                                              c->stream));
+        ADJ_CHECK(a, swe::launch_finalize_dt(a->partial, 1024, cfl, c->G.dx, c->G.dy, c->scal,
+                                             a->dts, a->counter, a->nsteps, c->stream));
+    }
+    const int rc = recompute_u1_u2(a, in, p);
+    if (rc) return rc;
+    const State u2{{c->U2[0], c->U2[1], c->U2[2]}};
+    ADJ_CHECK(a, swe::launch_stage(3, c->bc, false, false, c->G,
+                                   fwd_args(a, u2, in, out, a->dts + p), c->stream, nullptr));
+    return SWE_OK;
+}
+
+int advance(swe_hip_adj *a, const Action &act, double cfl) {
+    const State src = state_of(a, act.src), work = state_of(a, -1);
+    for (int64_t p = act.a; p < act.b; ++p) {
+        const State in = p == act.a ? src : work;
+        const State out = (p == act.b - 1 && act.dst >= 0) ? state_of(a, act.dst) : work;
+        const int rc = forward_step(a, in, out, p, cfl);
+        if (rc) return rc;
+    }
+    return SWE_OK;
+}
+
+swe::AdjArgs adj_args(swe_hip_adj *a, const State &v, int lam_in, int lam_out,
+                      const unsigned char *mask_out, unsigned char *mask_in, int64_t p) {
+    swe_hip_ctx *c = a->ctx;
+    swe::AdjArgs r{};
+    r.v_h = v.p[0];
+    r.v_hu = v.p[1];
+    r.v_hv = v.p[2];
+    r.B = c->B;
+    r.lam_h = a->lam[lam_in][0];
+    r.lam_hu = a->lam[lam_in][1];
+    r.lam_hv = a->lam[lam_in][2];
+    r.mask_out = mask_out;
+    r.out_h = a->lam[lam_out][0];
+    r.out_hu = a->lam[lam_out][1];
+    r.out_hv = a->lam[lam_out][2];
+    r.e_h = a->lam[3][0];
+    r.e_hu = a->lam[3][1];
+    r.e_hv = a->lam[3][2];
+    r.lam_b = a->lam_b;
+    r.mask_in = mask_in;
+    r.dt = a->dts + p;
+    r.tile_rows = a->tile_rows;
+    r.nstrips = swe::adj_strips_for(c->G.ny);
+    return r;
+}
+
+// Reverse step p from its start state in `src`: lam[0] = dL/dU^{p+1} -> dL/dU^p.
+int reverse_step(swe_hip_adj *a, int src, int64_t p) {
+    swe_hip_ctx *c = a->ctx;
+    const State base = state_of(a, src);
+    int rc = recompute_u1_u2(a, base, p);
+    if (rc) return rc;
+    const State u1{{c->U1[0], c->U1[1], c->U1[2]}}, u2{{c->U2[0], c->U2[1], c->U2[2]}};
+    unsigned char *m0 = a->mask[a->parity], *m1 = a->mask[1 - a->parity];
+    // stage 3: V = U2, output U^{p+1} (mask m0)
+    ADJ_CHECK(a, swe::launch_adjoint_stage(3, c->bc, c->G, adj_args(a, u2, 0, 1, m0, m1, p),
+                                           c->stream));
+    // stage 2: V = U1, output U2 (mask m1, written by stage 3)
+    ADJ_CHECK(a, swe::launch_adjoint_stage(2, c->bc, c->G, adj_args(a, u1, 1, 2, m1, m0, p),
+                                           c->stream));
+    // stage 1: V = U^p, output U1 (mask m0, written by stage 2); writes the mask of U^p
+    ADJ_CHECK(a, swe::launch_adjoint_stage(1, c->bc, c->G, adj_args(a, base, 2, 0, m0, m1, p),
+                                           c->stream));
+    a->parity = 1 - a->parity;
+    return SWE_OK;
+}
+
+int check_nonfinite(swe_hip_adj *a) {
+    int flag = 0;
+    ADJ_CHECK(a, hipMemcpyAsync(&flag, a->ctx->flag, sizeof(int), hipMemcpyDeviceToHost,
+                                a->ctx->stream));
+    ADJ_CHECK(a, hipStreamSynchronize(a->ctx->stream));
+    if (flag) return afail(a, SWE_ERR_NONFINITE, "non-finite state in the forward pass");
+    return SWE_OK;
+}
+
+int backward_common(swe_hip_adj *a, const double *lh, const double *lhu, const double *lhv,
+                    double *gh, double *ghu, double *ghv, double *gb, hipMemcpyKind in_kind,
+                    hipMemcpyKind out_kind) {
+    if (!a || !lh || !lhu || !lhv || !gh || !ghu || !ghv) return SWE_ERR_INVALID;
+    if (a->state != 1)
+        return afail(a, SWE_ERR_INVALID, "backward needs a forward pass with fresh checkpoints");
+    swe_hip_ctx *c = a->ctx;
+    ADJ_CHECK(a, hipSetDevice(c->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    const size_t bytes = a->n_total * sizeof(double);
+    const double *lin[3] = {lh, lhu, lhv};
+    for (int k = 0; k < 3; ++k) {
+        ADJ_CHECK(a, hipMemcpyAsync(a->lam[0][k], lin[k], bytes, in_kind, c->stream));
+        ADJ_CHECK(a, swe::launch_zero_ghosts(c->G, a->lam[0][k], c->stream));
+    }
+    ADJ_CHECK(a, swe::launch_fill_double(a->lam_b, 0.0, (long)a->n_total, c->stream));
+    a->parity = 0;
+    ADJ_CHECK(a, swe::launch_floor_mask(c->G, c->U[0], c->U[1], c->U[2], a->mask[0], c->stream));
+    a->state = 2;
+    for (size_t i = a->first_adj; i < a->acts.size(); ++i) {
+        const Action &act = a->acts[i];
+        const int rc = act.kind == kAdvance ? advance(a, act, 0.0) : reverse_step(a, act.src, act.a);
+        if (rc) return rc;
+    }
+    double *gout[3] = {gh, ghu, ghv};
+    for (int k = 0; k < 3; ++k)
+        ADJ_CHECK(a, hipMemcpyAsync(gout[k], a->lam[0][k], bytes, out_kind, c->stream));
+    if (gb) ADJ_CHECK(a, hipMemcpyAsync(gb, a->lam_b, bytes, out_kind, c->stream));
+    ADJ_CHECK(a, hipStreamSynchronize(c->stream));
+    a->bwd_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                    .count();
+    return check_nonfinite(a);
+}
+
+}  // namespace
+
+extern "C" {
+
+int swe_hip_adj_schedule(int64_t nsteps, int32_t nsnap, int64_t *advances, int32_t *max_reps,
+                         int64_t *actions) {
+    if (nsteps < 1 || nsnap < 1) return SWE_ERR_INVALID;
+    const std::vector<Action> acts = make_schedule(nsteps, nsnap);
+    std::vector<int32_t> reps((size_t)nsteps, 0);
+    int64_t adv = 0;
+    for (const Action &a : acts)
+        if (a.kind == kAdvance) {
+            adv += a.b - a.a;
+            for (int64_t p = a.a; p < a.b; ++p) ++reps[(size_t)p];
+        }
+    if (advances) *advances = adv;
+    if (max_reps) *max_reps = *std::max_element(reps.begin(), reps.end());
+    if (actions) *actions = (int64_t)acts.size();
+    return SWE_OK;
+}
+
+int swe_hip_adj_create(swe_hip_ctx *ctx, int64_t nsteps, int32_t nsnap, swe_hip_adj **out) {
+    if (!ctx || !out || nsteps < 1 || nsnap < 0) return SWE_ERR_INVALID;
+    *out = nullptr;
+    if (ctx->G.ng != 2) return SWE_ERR_INVALID;
+    if (hipSetDevice(ctx->device) != hipSuccess) return SWE_ERR_HIP;
+    auto *a = new swe_hip_adj();
+    a->ctx = ctx;
+    a->nsteps = nsteps;
+    a->n_total = ctx->n_total;
+    const size_t bytes = a->n_total * sizeof(double);
+    auto bad = [&](int code) {
+        swe_hip_adj_destroy(a);
+        return code;
+    };
+    for (int i = 0; i < 3; ++i)
+        if (hipMalloc(&a->W[i], bytes) != hipSuccess) return bad(SWE_ERR_OOM);
+    for (auto &l : a->lam)
+        for (auto &p : l)
+            if (hipMalloc(&p, bytes) != hipSuccess) return bad(SWE_ERR_OOM);
+    if (hipMalloc(&a->lam_b, bytes) != hipSuccess) return bad(SWE_ERR_OOM);
+    for (auto &m : a->mask)
+        if (hipMalloc(&m, a->n_total) != hipSuccess) return bad(SWE_ERR_OOM);
+    if (hipMalloc(&a->dts, sizeof(double) * (size_t)nsteps) != hipSuccess ||
+        hipMalloc(&a->partial, sizeof(double) * 1024) != hipSuccess ||
+        hipMalloc(&a->counter, sizeof(long long)) != hipSuccess)
+        return bad(SWE_ERR_OOM);
+    const size_t state_bytes = 3 * bytes;
+    if (nsnap == 0) {
+        size_t free_b = 0, total_b = 0;
+        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return bad(SWE_ERR_HIP);
+        const int64_t fit = (int64_t)(free_b / 2 / state_bytes);
+        nsnap = (int32_t)std::max<int64_t>(1, std::min<int64_t>({nsteps, fit, INT32_MAX}));
+    }
+    a->nsnap = nsnap;
+    if (hipMalloc(&a->slots, state_bytes * (size_t)nsnap) != hipSuccess) return bad(SWE_ERR_OOM);
+    a->acts = make_schedule(nsteps, nsnap);
+    for (size_t i = 0; i < a->acts.size(); ++i)
+        if (a->acts[i].kind == kAdjoint) {
+            a->first_adj = i;
+            break;
+        }
+    // taller tiles than the forward's: each tile re-reverses three halo faces
+    a->tile_rows = ctx->G.nx >= 512 ? 16 : 8;
+    *out = a;
+    return SWE_OK;
+}
+
+void swe_hip_adj_destroy(swe_hip_adj *a) {
+    if (!a) return;
+    (void)hipSetDevice(a->ctx->device);
+    (void)hipStreamSynchronize(a->ctx->stream);
+    (void)hipFree(a->slots);
+    for (auto *p : a->W) (void)hipFree(p);
+    for (auto &l : a->lam)
+        for (auto *p : l) (void)hipFree(p);
+    (void)hipFree(a->lam_b);
+    for (auto *m : a->mask) (void)hipFree(m);
+    (void)hipFree(a->dts);
+    (void)hipFree(a->partial);
+    (void)hipFree(a->counter);
+    delete a;
+}
+
+const char *swe_hip_adj_last_error(const swe_hip_adj *a) { return a ? a->err.c_str() : ""; }
+
+int swe_hip_adj_forward(swe_hip_adj *a, int32_t dt_mode, double value, double *dt_hist) {
+    if (!a) return SWE_ERR_INVALID;
+    if (dt_mode == SWE_DT_GIVEN ? dt_hist == nullptr
+                                : !(value > 0.0) || (dt_mode != SWE_DT_FIXED &&
+                                                     dt_mode != SWE_DT_ADAPTIVE))
+        return SWE_ERR_INVALID;
+    swe_hip_ctx *c = a->ctx;
+    ADJ_CHECK(a, hipSetDevice(c->device));
+    const auto t0 = std::chrono::steady_clock::now();
+    const size_t bytes = a->n_total * sizeof(double);
+    double cfl = 0.0;
+    if (dt_mode == SWE_DT_FIXED) {
+        ADJ_CHECK(a, swe::launch_fill_double(a->dts, value, (long)a->nsteps, c->stream));
+    } else if (dt_mode == SWE_DT_GIVEN) {
+        for (int64_t p = 0; p < a->nsteps; ++p)
+            if (!(dt_hist[p] > 0.0)) return afail(a, SWE_ERR_INVALID, "dt must be positive");
+        ADJ_CHECK(a, hipMemcpyAsync(a->dts, dt_hist, sizeof(double) * (size_t)a->nsteps,
+                                    hipMemcpyHostToDevice, c->stream));
+    } else {
+        cfl = value;
+        ADJ_CHECK(a, swe::launch_fill_int(reinterpret_cast<int *>(a->counter), 0, 2, c->stream));
+    }
+    ADJ_CHECK(a, swe::launch_fill_int(c->flag, 0, 1, c->stream));
+    // U^0 -> slot 0
+    for (int k = 0; k < 3; ++k)
+        ADJ_CHECK(a, hipMemcpyAsync(slot_ptr(a, 0, k), c->U[k], bytes, hipMemcpyDeviceToDevice,
+                                    c->stream));
+    // the schedule's first sweep: every action before the first reversal is an advance
+    for (size_t i = 0; i < a->first_adj; ++i) {
+        const int rc = advance(a, a->acts[i], cfl);
+        if (rc) return rc;
+    }
+    // the last step, from the start state the first reversal uses, into the context
+    const Action &first = a->acts[a->first_adj];
+    const State cu{{c->U[0], c->U[1], c->U[2]}};
+    int rc = forward_step(a, state_of(a, first.src), cu, a->nsteps - 1, cfl);
+    if (rc) return rc;
+    if (dt_hist && dt_mode != SWE_DT_GIVEN)
+        ADJ_CHECK(a, hipMemcpyAsync(dt_hist, a->dts, sizeof(double) * (size_t)a->nsteps,
+                                    hipMemcpyDeviceToHost, c->stream));
+    ADJ_CHECK(a, hipStreamSynchronize(c->stream));
+    a->fwd_ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                    .count();
+    rc = check_nonfinite(a);
+    if (rc) return rc;
+    a->state = 1;
+    return SWE_OK;
+}
+
+int swe_hip_adj_backward(swe_hip_adj *a, const double *lh, const double *lhu, const double *lhv,
+                         double *gh, double *ghu, double *ghv, double *gb) {
+    return backward_common(a, lh, lhu, lhv, gh, ghu, ghv, gb, hipMemcpyHostToDevice,
+                           hipMemcpyDeviceToHost);
+}
+
+int swe_hip_adj_backward_device(swe_hip_adj *a, const double *lh, const double *lhu,
+                                const double *lhv, double *gh, double *ghu, double *ghv,
+                                double *gb) {
+    return backward_common(a, lh, lhu, lhv, gh, ghu, ghv, gb, hipMemcpyDeviceToDevice,
+                           hipMemcpyDeviceToDevice);
+}
+
+int swe_hip_adj_set_tile_rows(swe_hip_adj *a, int tile_rows) {
+    if (!a || tile_rows < 1) return SWE_ERR_INVALID;
+    a->tile_rows = tile_rows;
+    return SWE_OK;
+}
+
+int swe_hip_adj_info(const swe_hip_adj *a, int32_t *nsnap, double *forward_ms,
+                     double *backward_ms) {
+    if (!a) return SWE_ERR_INVALID;
+    if (nsnap) *nsnap = a->nsnap;
+    if (forward_ms) *forward_ms = a->fwd_ms;
+    if (backward_ms) *backward_ms = a->bwd_ms;
+    return SWE_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,41 @@
+// swe_ctx.h -- the single-grid context behind the C ABI (shared by the step and the
+// adjoint entry points).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+
+#include "../../include/swe_hip.h"
+#include "swe_internal.h"
+
+struct swe_hip_ctx {
+    int device = 0;
+    hipStream_t own_stream = nullptr;
+    hipStream_t stream = nullptr;
+    swe::GridDev G{};
+    int bc = 0;
+    size_t n_total = 0;
+    double *U[3] = {nullptr, nullptr, nullptr};
+    double *U1[3] = {nullptr, nullptr, nullptr};
+    double *U2[3] = {nullptr, nullptr, nullptr};
+    double *B = nullptr;
+    double *scal = nullptr;      // [0] dt, [1] max speed
+    double *partial = nullptr;   // per-block maxima
+    int partial_cap = 0;
+    int *flag = nullptr;
+    long long *counter = nullptr;
+    double *hist = nullptr;
+    long long hist_cap = 0;
+    int tile_rows = 64;
+    int precision = SWE_PRECISION_EXACT;
+    bool use_graph = true;
+    // cached graph of one step: key = (mode, tile_rows, stream, hist pointer)
+    hipGraphExec_t graph = nullptr;
+    int graph_mode = -1;
+    int graph_tile = -1;
+    int graph_precision = -1;
+    hipStream_t graph_stream = nullptr;
+    double graph_cfl = 0.0;
+    std::string err;
+};

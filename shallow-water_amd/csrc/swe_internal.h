@@ -103,4 +103,37 @@ hipError_t launch_set_scalar(double *dst, double v, hipStream_t stream);
 hipError_t launch_fill_int(int *dst, int v, int n, hipStream_t stream);
 hipError_t launch_fill_double(double *dst, double v, long n, hipStream_t stream);
 
+// ---- discrete adjoint ------------------------------------------------------------------
+// Arguments of one adjoint stage launch (stage s of a step, reversed).
+//   v_*      : the stage input V (U^n, U1 or U2), read through the stencil
+//   lam_*    : dL/d(stage output), interior cells
+//   mask_out : 1 where the stage output is the floor state (1e-8, 0, 0)
+//   out_*    : dL/dV (stage 1: dL/dU^n, the e_* accumulator included)
+//   e_*      : dL/dU^n accumulator (stage 3 writes, stage 2 adds, stage 1 reads)
+//   lam_b    : dL/dB accumulator (every stage adds)
+//   mask_in  : written: 1 where V is the floor state (mask_out of the previous stage's adjoint)
+struct AdjArgs {
+    const double *v_h, *v_hu, *v_hv;
+    const double *B;
+    const double *lam_h, *lam_hu, *lam_hv;
+    const unsigned char *mask_out;
+    double *out_h, *out_hu, *out_hv;
+    double *e_h, *e_hu, *e_hv;
+    double *lam_b;
+    unsigned char *mask_in;
+    const double *dt;  // device scalar: this step's dt (frozen)
+    int tile_rows;     // interior rows per wave
+    int nstrips;       // 58-column strips across y (adj_strips_for)
+};
+
+int adj_strips_for(int ny);
+// ng must be 2.
+hipError_t launch_adjoint_stage(int stage, int bc, const GridDev &G, const AdjArgs &A,
+                                hipStream_t stream);
+// mask[k] = 1 where (h, hu, hv)[k] is the floor state, 0 elsewhere (interior cells only).
+hipError_t launch_floor_mask(const GridDev &G, const double *h, const double *hu,
+                             const double *hv, unsigned char *mask, hipStream_t stream);
+// Zero the ghost frame of one array.
+hipError_t launch_zero_ghosts(const GridDev &G, double *a, hipStream_t stream);
+
 }  // namespace swe

@@ -1,0 +1,65 @@
+// swe_lanes.h -- cross-lane moves, ghost-index mirroring and row-based addressing shared by
+// the stage kernel and its adjoint.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace swe {
+
+// Lane l receives lane l+1's value / lane l-1's value, through the DPP wavefront shifts
+// wave_shl:1 (0x130) and wave_shr:1 (0x138) of the GFX9 ISA: a VALU move, no LDS round
+// trip.  The lane without a source (63 resp. 0) keeps its own value; callers only consume
+// the results of lanes that have a source.
+template <int CTRL>
+__device__ __forceinline__ double dpp_move(double x) {
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(x), __double2loint(x), CTRL, 0xF,
+                                               0xF, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(x), __double2hiint(x), CTRL, 0xF,
+                                               0xF, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double from_next(double x) { return dpp_move<0x130>(x); }
+__device__ __forceinline__ double from_prev(double x) { return dpp_move<0x138>(x); }
+
+// Value of lane `src` (wave-uniform) in every lane.
+__device__ __forceinline__ double read_lane(double x, int src) {
+    const int lo = __builtin_amdgcn_readlane(__double2loint(x), src);
+    const int hi = __builtin_amdgcn_readlane(__double2hiint(x), src);
+    return __hiloint2double(hi, lo);
+}
+
+// Source index of row/column r for the interior range lo..hi-1: r itself inside, the
+// mirrored cell 2*lo-1-r / 2*hi-1-r for reflecting walls (BC 0), the nearest interior cell
+// for transmissive ones (BC 1).  r may lie up to 2 cells outside the interior.
+template <int BC>
+__device__ __forceinline__ int mirror(int r, int lo, int hi, bool &ghost) {
+    ghost = false;
+    if (r < lo) {
+        ghost = true;
+        return BC == 0 ? 2 * lo - 1 - r : lo;
+    }
+    if (r >= hi) {
+        ghost = true;
+        return BC == 0 ? 2 * hi - 1 - r : hi - 1;
+    }
+    return r;
+}
+
+// The row offset is wave-uniform and the column is fixed per lane, so every access is a
+// scalar row base plus a constant 32-bit per-lane byte offset: the global_load/store
+// "saddr + voffset" form, with no per-row 64-bit address arithmetic on the vector ALUs.
+// row_off = instance offset (0 for a single grid) + row * stride, wave-uniform.
+__device__ __forceinline__ const double *row_ptr(const double *base, long long row_off) {
+    return base + row_off;
+}
+__device__ __forceinline__ double *row_ptr(double *base, long long row_off) {
+    return base + row_off;
+}
+__device__ __forceinline__ double ld(const double *row, unsigned byte_off) {
+    return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(row) + byte_off);
+}
+__device__ __forceinline__ void st(double *row, unsigned byte_off, double v) {
+    *reinterpret_cast<double *>(reinterpret_cast<char *>(row) + byte_off) = v;
+}
+
+}  // namespace swe

@@ -30,6 +30,10 @@ BC_TRANSMISSIVE = 1
 PRECISION_EXACT = 0   # bitwise the reference's arithmetic
 PRECISION_FAST = 1    # reciprocal-based velocities, few-ulp differences per step
 
+DT_FIXED = 0      # one dt for every step
+DT_ADAPTIVE = 1   # dt from the CFL number at each step's start state (recorded)
+DT_GIVEN = 2      # a caller-given dt per step
+
 G_REF = 9.80665  # reference include/constants.hpp:4
 
 _dp = C.POINTER(C.c_double)
@@ -92,6 +96,16 @@ _SIGS = {
     "swe_hip_batch_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
     "swe_hip_batch_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "swe_hip_batch_sync": (C.c_int, [C.c_void_p]),
+    "swe_hip_adj_create": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.POINTER(C.c_void_p)]),
+    "swe_hip_adj_destroy": (None, [C.c_void_p]),
+    "swe_hip_adj_last_error": (C.c_char_p, [C.c_void_p]),
+    "swe_hip_adj_forward": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, _dp]),
+    "swe_hip_adj_backward": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
+    "swe_hip_adj_backward_device": (C.c_int, [C.c_void_p] + [C.c_void_p] * 7),
+    "swe_hip_adj_set_tile_rows": (C.c_int, [C.c_void_p, C.c_int]),
+    "swe_hip_adj_schedule": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_int64),
+                                       C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
+    "swe_hip_adj_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), _dp, _dp]),
 }
 
 _lib = None
@@ -101,6 +115,14 @@ def lib():
     """Load libswe_hip.so (raises if it has not been built)."""
     global _lib
     if _lib is None:
+        # One HIP runtime per process.  torch ships its own libamdhip64; if this library
+        # (linked against the system ROCm) initializes HIP first, torch's runtime then finds
+        # no GPU.  Importing torch first makes the library resolve to torch's runtime, so
+        # both share the device (and torch tensors can be passed as device pointers).
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"{LIB_PATH} is missing: build it with `python -m swe_hip.build` "
                                "(or __graft_entry__.build())")
@@ -341,3 +363,91 @@ class HipBatch:
 
     def sync(self):
         self._check(lib().swe_hip_batch_sync(self._b))
+
+
+def adjoint_schedule(nsteps: int, nsnap: int):
+    """(forward steps advanced in all, most advances of one step, schedule actions) of the
+    binomial checkpoint schedule; host-only, needs no GPU."""
+    adv, reps, acts = C.c_int64(), C.c_int32(), C.c_int64()
+    rc = lib().swe_hip_adj_schedule(int(nsteps), int(nsnap), C.byref(adv), C.byref(reps),
+                                    C.byref(acts))
+    if rc != SWE_OK:
+        raise SweError(rc, lib().swe_hip_error_string(rc).decode())
+    return adv.value, reps.value, acts.value
+
+
+class HipAdjoint:
+    """Discrete adjoint of `nsteps` SSP-RK3 steps of a HipSolver (swe_hip_adj_* in the C ABI).
+
+    forward() runs the steps from the solver's current state, storing binomial checkpoints
+    (nsnap stored states, U^0 included; 0 = as many as half the free device memory holds,
+    at most nsteps) and leaves U^N in the solver; backward(lam) takes dL/dU^N and returns
+    dL/dU^0 and dL/dB with dt frozen from the forward pass.  One backward per forward."""
+
+    def __init__(self, solver: HipSolver, nsteps: int, nsnap: int = 0):
+        self.solver = solver
+        self.nsteps = int(nsteps)
+        self._a = C.c_void_p()
+        self._check(lib().swe_hip_adj_create(solver._ctx, self.nsteps, int(nsnap),
+                                             C.byref(self._a)))
+
+    def _check(self, rc):
+        if rc != SWE_OK:
+            L = lib()
+            detail = (L.swe_hip_adj_last_error(self._a) or b"").decode() if self._a else ""
+            raise SweError(rc, L.swe_hip_error_string(rc).decode() + (f": {detail}" if detail else ""))
+
+    def close(self):
+        if self._a:
+            lib().swe_hip_adj_destroy(self._a)
+            self._a = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def forward(self, dt=None, cfl=None, dts=None):
+        """Exactly one of: dt (fixed), cfl (adaptive, recorded), dts (given per step).
+        Returns the dt sequence used."""
+        hist = np.zeros(self.nsteps)
+        if dts is not None:
+            hist[:] = np.asarray(dts, dtype=np.float64)
+            self._check(lib().swe_hip_adj_forward(self._a, DT_GIVEN, 0.0, _p(hist)))
+        elif cfl is not None:
+            self._check(lib().swe_hip_adj_forward(self._a, DT_ADAPTIVE, float(cfl), _p(hist)))
+        else:
+            self._check(lib().swe_hip_adj_forward(self._a, DT_FIXED, float(dt), _p(hist)))
+        return hist
+
+    def backward(self, lam_h, lam_hu, lam_hv):
+        lam = [self.solver._arr(a) for a in (lam_h, lam_hu, lam_hv)]
+        out = [np.empty(self.solver.shape) for _ in range(4)]
+        self._check(lib().swe_hip_adj_backward(self._a, *(_p(a) for a in lam),
+                                               *(_p(o) for o in out)))
+        return out  # dL/dh0, dL/dhu0, dL/dhv0, dL/dB
+
+    def backward_device(self, lam_h, lam_hu, lam_hv, g_h, g_hu, g_hv, g_b=None):
+        """torch float64 device tensors (contiguous, grid shape)."""
+        ts = [lam_h, lam_hu, lam_hv, g_h, g_hu, g_hv] + ([g_b] if g_b is not None else [])
+        for t in ts:
+            if tuple(t.shape) != self.solver.shape or not t.is_contiguous():
+                raise ValueError("device tensors must be contiguous with the grid shape")
+        self._check(lib().swe_hip_adj_backward_device(
+            self._a, lam_h.data_ptr(), lam_hu.data_ptr(), lam_hv.data_ptr(), g_h.data_ptr(),
+            g_hu.data_ptr(), g_hv.data_ptr(), g_b.data_ptr() if g_b is not None else None))
+
+    def set_tile_rows(self, rows: int):
+        self._check(lib().swe_hip_adj_set_tile_rows(self._a, int(rows)))
+
+    def info(self):
+        n, f, b = C.c_int32(), C.c_double(), C.c_double()
+        self._check(lib().swe_hip_adj_info(self._a, C.byref(n), C.byref(f), C.byref(b)))
+        return {"nsnap": n.value, "forward_ms": f.value, "backward_ms": b.value}

@@ -22,7 +22,8 @@ CSRC = os.path.join(PKG_ROOT, "csrc")
 REPO = os.path.dirname(PKG_ROOT)
 LIB = os.path.join(HERE, "libswe_hip.so")
 
-SOURCES = ["swe_stage.hip", "swe_api.hip", "swe_batch_api.hip"]
+SOURCES = ["swe_stage.hip", "swe_api.hip", "swe_batch_api.hip", "swe_adjoint.hip",
+           "swe_adjoint_api.hip"]
 ARCH = os.environ.get("SWE_HIP_ARCH", "gfx950")
 
 HIPCC_FLAGS = [

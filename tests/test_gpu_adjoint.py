@@ -1,0 +1,225 @@
+"""Discrete adjoint on the GPU (swe_hip_adj_* through the C ABI).
+
+The adjoint is pinned by the CPU dual-number tangent of the serial solver
+(oracle/swe_tangent.cpp), which uses the same one-sided derivative conventions: for random
+output weights lam and input directions v, <lam, J v> (tangent) must equal <J^T lam, v>
+(adjoint) to rounding.  The forward pass under the adjoint object must equal the plain
+solver bitwise, and the gradient must not depend on the number of stored checkpoints.
+"""
+import numpy as np
+import pytest
+
+import oracle
+from swe_hip import workloads as W
+
+pytestmark = pytest.mark.gpu
+
+
+def bits(a):
+    return np.ascontiguousarray(a).view(np.uint64)
+
+
+def same(a, b):
+    return np.array_equal(bits(a), bits(b))
+
+
+def interior(shape, rng, ng=2, scale=1.0):
+    a = np.zeros(shape)
+    a[ng:-ng, ng:-ng] = scale * rng.standard_normal((shape[0] - 2 * ng, shape[1] - 2 * ng))
+    return a
+
+
+def dry_dam(nx, ny):
+    """Dam break onto a dry bed (h = 1e-8, the floor state) over a flat bottom."""
+    c = W._case(nx, ny, 1.0, 1.0, 9.80665, 0, 0.3, 1, 2, name=f"drydam_{nx}x{ny}")
+    x = (np.arange(nx) + 0.5) / nx
+    c["h"][2:-2, 2:-2] = np.where(x[:, None] < 0.4, 1.0, 1e-8) * np.ones((1, ny))
+    W.fill_ghosts(c["h"], 2, 0)
+    return c
+
+
+def setup(case, bc):
+    gr = oracle.grid(case["nx"], case["ny"], 2, case["dx"], case["dy"], g=case["g"], bc=bc)
+    B = case["B"].copy()
+    oracle.apply_bc(gr, B)
+    dt, _ = oracle.stable_dt(gr, case["h"], case["hu"], case["hv"], 0.3)
+    return gr, B, dt
+
+
+def gpu_gradient(hip, case, bc, nsteps, dt, lam, nsnap=2, tile_rows=None):
+    c = dict(case)
+    c["bc"] = bc
+    with hip.HipSolver.from_case(c) as s:
+        s.upload(case["h"], case["hu"], case["hv"], case["B"])
+        with hip.HipAdjoint(s, nsteps, nsnap) as adj:
+            if tile_rows:
+                adj.set_tile_rows(tile_rows)
+            adj.forward(dt=dt)
+            final = s.download()
+            grads = adj.backward(*lam)
+    return final, grads
+
+
+def mild_state(nx, ny, seed, bc=0, bathy_scale=0.15):
+    """Rough random state whose bottom variation stays well below the depth, so every
+    hydrostatic face depth is bounded away from 0 and the Jacobian is well conditioned."""
+    case = W.random_state(nx, ny, seed=seed, bathy=True, bc=bc)
+    case["B"] = np.ascontiguousarray(case["B"] * bathy_scale)
+    return case
+
+
+CASES = [
+    # (nx, ny, nsteps, bc, tile_rows, bathy_scale, rel_tol): multi-strip (ny > 58), tiles
+    # shorter than the grid, single-row tiles, tiny grids.  The last case keeps the full
+    # random bottom: nearly dry hydrostatic faces make the Jacobian huge (|J| ~ 1e40), so
+    # only its rounding-level agreement bound is looser.
+    (20, 17, 1, 0, None, 0.15, 1e-10),
+    (40, 70, 3, 0, 8, 0.15, 1e-10),
+    (33, 130, 2, 1, 5, 0.15, 1e-10),
+    (70, 61, 4, 0, 16, 0.15, 1e-10),
+    (9, 7, 3, 1, 3, 0.15, 1e-10),
+    (12, 120, 2, 0, 1, 0.15, 1e-10),
+    (70, 61, 4, 0, 16, 1.0, 1e-8),
+]
+
+
+@pytest.mark.parametrize("nx,ny,nsteps,bc,tile,bscale,tol", CASES)
+def test_adjoint_matches_tangent(hip, nx, ny, nsteps, bc, tile, bscale, tol):
+    case = mild_state(nx, ny, nx * 7 + ny, bc, bscale)
+    gr, B, dt = setup(case, bc)
+    rng = np.random.default_rng(nx + 100 * ny)
+    shape = case["h"].shape
+    lam = [interior(shape, rng) for _ in range(3)]
+    final, (gh, ghu, ghv, gb) = gpu_gradient(hip, case, bc, nsteps, dt, lam, tile_rows=tile)
+
+    # forward under the adjoint object == the oracle's run, bitwise
+    ref = [case[k].copy() for k in ("h", "hu", "hv")]
+    rc, _, _ = oracle.run(gr, *ref, B, nsteps, dt=dt)
+    assert rc == 0
+    for g_, w_ in zip(final, ref):
+        assert same(g_, w_)
+
+    # ghost entries of the gradients are zero
+    for g_ in (gh, ghu, ghv, gb):
+        assert not g_[:2].any() and not g_[-2:].any() and not g_[:, :2].any() and not g_[:, -2:].any()
+
+    dts = np.full(nsteps, dt)
+    grads = (gh, ghu, ghv, gb)
+    errs = []
+    # one direction per input (h, hu, hv, B), then a mixed one
+    for k in range(5):
+        v = [interior(shape, rng) if (k == 4 or i == k) else np.zeros(shape) for i in range(4)]
+        x = [case[q].copy() for q in ("h", "hu", "hv")]
+        d = [a.copy() for a in v[:3]]
+        assert oracle.tangent_run(gr, *x, B, *d, v[3], dts) == 0
+        lhs_terms = [lam[i] * d[i] for i in range(3)]
+        lhs = sum(t.sum() for t in lhs_terms)
+        rhs = sum((g_ * v_).sum() for g_, v_ in zip(grads, v))
+        scale = sum(np.abs(t).sum() for t in lhs_terms)
+        errs.append(abs(lhs - rhs) / scale)
+    print("relative dot-product errors (h, hu, hv, B, mixed):", errs)
+    assert max(errs) <= tol, errs
+
+
+def test_adjoint_floor_mask_matches_tangent(hip):
+    case = dry_dam(40, 66)
+    gr, B, dt = setup(case, 0)
+    rng = np.random.default_rng(5)
+    shape = case["h"].shape
+    lam = [interior(shape, rng) for _ in range(3)]
+    nsteps = 4
+    _, (gh, ghu, ghv, gb) = gpu_gradient(hip, case, 0, nsteps, dt, lam)
+    dts = np.full(nsteps, dt)
+    for k in range(2):
+        v = [interior(shape, rng) for _ in range(4)]
+        x = [case[q].copy() for q in ("h", "hu", "hv")]
+        d = [a.copy() for a in v[:3]]
+        assert oracle.tangent_run(gr, *x, B, *d, v[3], dts) == 0
+        floored = (x[0] == 1e-8) & (x[1] == 0) & (x[2] == 0)
+        assert floored[2:-2, 2:-2].sum() > 0  # the case exercises the floor mask
+        lhs = sum((lam[i] * d[i]).sum() for i in range(3))
+        rhs = (gh * v[0]).sum() + (ghu * v[1]).sum() + (ghv * v[2]).sum() + (gb * v[3]).sum()
+        scale = sum(np.abs(lam[i] * d[i]).sum() for i in range(3))
+        assert abs(lhs - rhs) <= 1e-11 * scale, (lhs, rhs)
+
+
+def test_gradient_independent_of_checkpoints(hip):
+    case = mild_state(30, 64, 11)
+    _, _, dt = setup(case, 0)
+    rng = np.random.default_rng(3)
+    lam = [interior(case["h"].shape, rng) for _ in range(3)]
+    ref = None
+    for nsnap in (9, 1, 2, 3, 5):
+        _, g = gpu_gradient(hip, case, 0, 9, dt, lam, nsnap=nsnap)
+        if ref is None:
+            ref = g
+        else:
+            for a, b in zip(g, ref):
+                assert same(a, b), nsnap
+
+
+def test_gradient_independent_of_tile_rows(hip):
+    case = mild_state(50, 61, 12, bc=1)
+    _, _, dt = setup(case, 1)
+    rng = np.random.default_rng(4)
+    lam = [interior(case["h"].shape, rng) for _ in range(3)]
+    _, ref = gpu_gradient(hip, case, 1, 3, dt, lam, tile_rows=16)
+    for t in (1, 2, 7, 64):
+        _, g = gpu_gradient(hip, case, 1, 3, dt, lam, tile_rows=t)
+        for a, b in zip(g, ref):
+            assert same(a, b), t
+
+
+def test_adjoint_adaptive_dt_is_recorded_and_frozen(hip):
+    case = W.adjoint_case(48, steps=12)
+    with hip.HipSolver.from_case(case) as s:
+        s.upload(case["h"], case["hu"], case["hv"], case["B"])
+        want_hist = s.step_adaptive(0.4, 12)
+        want = s.download()
+    with hip.HipSolver.from_case(case) as s:
+        s.upload(case["h"], case["hu"], case["hv"], case["B"])
+        with hip.HipAdjoint(s, 12, 4) as adj:
+            hist = adj.forward(cfl=0.4)
+            got = s.download()
+    assert same(hist, want_hist)
+    for a, b in zip(got, want):
+        assert same(a, b)
+
+
+def test_backward_requires_fresh_forward(hip):
+    case = W.random_state(12, 10, seed=1)
+    _, _, dt = setup(case, 0)
+    lam = [np.zeros(case["h"].shape)] * 3
+    with hip.HipSolver.from_case(case) as s:
+        s.upload(case["h"], case["hu"], case["hv"], case["B"])
+        with hip.HipAdjoint(s, 2, 2) as adj:
+            with pytest.raises(hip.SweError):
+                adj.backward(*lam)
+            adj.forward(dt=dt)
+            adj.backward(*lam)
+            with pytest.raises(hip.SweError):
+                adj.backward(*lam)
+
+
+def test_torch_autograd_wrapper(hip):
+    torch = pytest.importorskip("torch")
+    from swe_hip.autograd import swe_rollout
+
+    case = mild_state(24, 70, 21)
+    _, _, dt = setup(case, 0)
+    rng = np.random.default_rng(8)
+    lam = [interior(case["h"].shape, rng) for _ in range(3)]
+    final, (gh, ghu, ghv, gb) = gpu_gradient(hip, case, 0, 5, dt, lam, nsnap=3)
+
+    dev = torch.device("cuda:0")
+    t = {k: torch.tensor(case[k], dtype=torch.float64, device=dev, requires_grad=True)
+         for k in ("h", "hu", "hv", "B")}
+    grid = {"nx": case["nx"], "ny": case["ny"], "dx": case["dx"], "dy": case["dy"],
+            "g": case["g"], "bc": 0}
+    hN, huN, hvN = swe_rollout(t["h"], t["hu"], t["hv"], t["B"], grid, 5, dt=dt, nsnap=3)
+    for a, b in zip((hN, huN, hvN), final):
+        assert same(a.detach().cpu().numpy(), b)
+    L = sum((torch.tensor(l, device=dev) * o).sum() for l, o in zip(lam, (hN, huN, hvN)))
+    L.backward()
+    for k, want in zip(("h", "hu", "hv", "B"), (gh, ghu, ghv, gb)):
+        assert same(t[k].grad.cpu().numpy(), want), k
