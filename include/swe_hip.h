@@ -204,8 +204,10 @@ typedef enum {
     SWE_DT_GIVEN = 2     /* a caller-given dt per step */
 } swe_dt_mode;
 
-/* nsnap = number of stored states, U^0 included (>= 1); 0 picks min(nsteps, what half of
- * the free device memory holds). */
+/* nsnap = number of stored states, U^0 included (>= 1).  -1 stores every step together with
+ * its two stage states (the backward then recomputes nothing; 9 arrays per step).  0 picks
+ * -1 when that fits in 60% of the free device memory, else as many states as half of it
+ * holds (at most nsteps). */
 int swe_hip_adj_create(swe_hip_ctx *ctx, int64_t nsteps, int32_t nsnap, swe_hip_adj **out);
 void swe_hip_adj_destroy(swe_hip_adj *adj);
 const char *swe_hip_adj_last_error(const swe_hip_adj *adj);
@@ -226,8 +228,9 @@ int swe_hip_adj_backward_device(swe_hip_adj *adj, const double *lam_h, const dou
                                 const double *lam_hv, double *grad_h, double *grad_hu,
                                 double *grad_hv, double *grad_b);
 
-/* Rows per wave of the adjoint stage kernel (>= 1). */
-int swe_hip_adj_set_tile_rows(swe_hip_adj *adj, int tile_rows);
+/* Launch shape of the adjoint stage kernels: x faces per wave of the x-face kernel and rows
+ * per wave of the y-face kernel (each >= 1; 0 keeps the current value). */
+int swe_hip_adj_set_launch(swe_hip_adj *adj, int x_rows, int y_rows);
 
 /* The checkpoint schedule for (nsteps, nsnap), computed on the host (no device needed):
  * total forward steps advanced (the first sweep included), the most times any one step is
@@ -235,10 +238,10 @@ int swe_hip_adj_set_tile_rows(swe_hip_adj *adj, int tile_rows);
 int swe_hip_adj_schedule(int64_t nsteps, int32_t nsnap, int64_t *advances, int32_t *max_reps,
                          int64_t *actions);
 
-/* Wall-clock milliseconds of the last forward and backward calls, and the number of stored
- * states in use. */
-int swe_hip_adj_info(const swe_hip_adj *adj, int32_t *nsnap, double *forward_ms,
-                     double *backward_ms);
+/* The number of stored states in use, whether stage states are stored too (1/0), and the
+ * wall-clock milliseconds of the last forward and backward calls.  Any pointer may be NULL. */
+int swe_hip_adj_info(const swe_hip_adj *adj, int32_t *nsnap, int32_t *records,
+                     double *forward_ms, double *backward_ms);
 
 #ifdef __cplusplus
 }

@@ -171,21 +171,25 @@ struct Solver {
     }
     void rhs(const T *h, const T *hu, const T *hv) {
         const int nG = gr.ng, s = gr.s;
-        T o[6];
+#pragma omp parallel for schedule(static)
         for (int i = nG - 1; i < nG + gr.nx; ++i)
             for (int j = nG; j < nG + gr.ny; ++j) {
+                T o[6];
                 const long k = (long)i * s + j;
                 face(gr, true, h, hu, hv, B.data(), k, s, o);
                 for (int c = 0; c < 6; ++c) f[c][k] = o[c];
             }
+#pragma omp parallel for schedule(static)
         for (int i = nG; i < nG + gr.nx; ++i)
             for (int j = nG - 1; j < nG + gr.ny; ++j) {
+                T o[6];
                 const long k = (long)i * s + j;
                 face(gr, false, h, hu, hv, B.data(), k, 1, o);
                 for (int c = 0; c < 6; ++c) f[6 + c][k] = o[c];
             }
         const double idx = 1.0 / gr.dx, idy = 1.0 / gr.dy;
         T *r[3] = {r0.data(), r1.data(), r2.data()};
+#pragma omp parallel for schedule(static)
         for (int i = nG; i < nG + gr.nx; ++i)
             for (int j = nG; j < nG + gr.ny; ++j) {
                 const long k = (long)i * s + j;
@@ -195,6 +199,7 @@ struct Solver {
     }
     bool floor_(T *h, T *hu, T *hv) {
         bool bad = false;
+#pragma omp parallel for schedule(static) reduction(|| : bad)
         for (int i = gr.ng; i < gr.ng + gr.nx; ++i)
             for (int j = gr.ng; j < gr.ng + gr.ny; ++j) {
                 const long k = (long)i * gr.s + j;
@@ -213,12 +218,14 @@ struct Solver {
         bool bad = false;
         ghost(gr, h, hu, hv);
         rhs(h, hu, hv);
+#pragma omp parallel for schedule(static)
         for (long k = 0; k < n; ++k) {
             h1[k] = h[k] + dt * r0[k]; hu1[k] = hu[k] + dt * r1[k]; hv1[k] = hv[k] + dt * r2[k];
         }
         bad |= floor_(h1.data(), hu1.data(), hv1.data());
         ghost(gr, h1.data(), hu1.data(), hv1.data());
         rhs(h1.data(), hu1.data(), hv1.data());
+#pragma omp parallel for schedule(static)
         for (long k = 0; k < n; ++k) {
             h2[k] = 0.75 * h[k] + 0.25 * (h1[k] + dt * r0[k]);
             hu2[k] = 0.75 * hu[k] + 0.25 * (hu1[k] + dt * r1[k]);
@@ -227,6 +234,7 @@ struct Solver {
         bad |= floor_(h2.data(), hu2.data(), hv2.data());
         ghost(gr, h2.data(), hu2.data(), hv2.data());
         rhs(h2.data(), hu2.data(), hv2.data());
+#pragma omp parallel for schedule(static)
         for (long k = 0; k < n; ++k) {
             h[k] = (1.0 / 3.0) * h[k] + (2.0 / 3.0) * (h2[k] + dt * r0[k]);
             hu[k] = (1.0 / 3.0) * hu[k] + (2.0 / 3.0) * (hu2[k] + dt * r1[k]);

@@ -91,11 +91,13 @@ __device__ __forceinline__ void slope_parts(int sel, double sb, double &to_prev,
 }
 
 // Velocities of a reconstructed state exactly as the forward computes them.
+// In split mode r receives the refined reciprocal of h the division used (the reverse
+// reuses it); otherwise r is left for the reverse to compute.
 template <bool SPLIT>
 __device__ __forceinline__ void velocities(double n, double t, double h, double &un,
-                                           double &ut) {
+                                           double &ut, double &r) {
     if constexpr (SPLIT) {
-        const double r = div_denominator(h);
+        r = div_denominator(h);
         const double qn = div_numerator(n, h, r), qt = div_numerator(t, h, r);
         un = h > kEps ? qn : 0.0;
         ut = h > kEps ? qt : 0.0;
@@ -103,6 +105,13 @@ __device__ __forceinline__ void velocities(double n, double t, double h, double 
         un = safe_velocity(n, h);
         ut = safe_velocity(t, h);
     }
+}
+
+// 1/h for the reverse: the primal's reciprocal in split mode, a refined one otherwise.
+template <bool SPLIT>
+__device__ __forceinline__ double inv_of(double h, double r_primal) {
+    if constexpr (SPLIT) return r_primal;
+    return recip(h);
 }
 
 struct FaceAdj {
@@ -125,15 +134,15 @@ __device__ __forceinline__ FaceAdj face_adjoint(double half_g, double g, double 
     const bool bstar_p = b_m < b_p;  // std::max returned its second argument
     const double xm = eta_m - b_m, xp = eta_p - b_p;
     const double h_m = positive_height(xm), h_p = positive_height(xp);
-    double un_m, ut_m, un_p, ut_p;
-    velocities<SPLIT>(n_m, t_m, h_m, un_m, ut_m);
-    velocities<SPLIT>(n_p, t_p, h_p, un_p, ut_p);
+    double un_m, ut_m, un_p, ut_p, r_m = 0.0, r_p = 0.0, r_L = 0.0, r_R = 0.0;
+    velocities<SPLIT>(n_m, t_m, h_m, un_m, ut_m, r_m);
+    velocities<SPLIT>(n_p, t_p, h_p, un_p, ut_p, r_p);
     const double ym = eta_m - b_star, yp = eta_p - b_star;
     const double hL = positive_height(ym), hR = positive_height(yp);
     const double nL = hL * un_m, tL = hL * ut_m, nR = hR * un_p, tR = hR * ut_p;
     double unL, utL, unR, utR;
-    velocities<SPLIT>(nL, tL, hL, unL, utL);
-    velocities<SPLIT>(nR, tR, hR, unR, utR);
+    velocities<SPLIT>(nL, tL, hL, unL, utL, r_L);
+    velocities<SPLIT>(nR, tR, hR, unR, utR, r_R);
     const double cL = ieee_sqrt<SPLIT>(g * hL), cR = ieee_sqrt<SPLIT>(g * hR);
     const double a1 = unL - cL, a2 = unR - cR, b1 = unL + cL, b2 = unR + cR;
     const double sL = fmin(a1, a2), sR = fmax(b1, b2);
@@ -210,13 +219,13 @@ __device__ __forceinline__ FaceAdj face_adjoint(double half_g, double g, double 
     if (cR > 0.0) bhR += bcR * (0.5 * g) * recip(cR);
     // HLL velocities
     if (hL > kEps) {
-        const double r = recip(hL);
+        const double r = inv_of<SPLIT>(hL, r_L);
         bnL += bunL * r;
         btL += butL * r;
         bhL -= (bunL * unL + butL * utL) * r;
     }
     if (hR > kEps) {
-        const double r = recip(hR);
+        const double r = inv_of<SPLIT>(hR, r_R);
         bnR += bunR * r;
         btR += butR * r;
         bhR -= (bunR * unR + butR * utR) * r;
@@ -239,13 +248,13 @@ __device__ __forceinline__ FaceAdj face_adjoint(double half_g, double g, double 
     }
     // face velocities u = n / h
     if (h_m > kEps) {
-        const double q = recip(h_m);
+        const double q = inv_of<SPLIT>(h_m, r_m);
         r.nm = bun_m * q;
         r.tm = but_m * q;
         bh_m -= (bun_m * un_m + but_m * ut_m) * q;
     }
     if (h_p > kEps) {
-        const double q = recip(h_p);
+        const double q = inv_of<SPLIT>(h_p, r_p);
         r.np = bun_p * q;
         r.tp = but_p * q;
         bh_p -= (bun_p * un_p + but_p * ut_p) * q;
@@ -278,25 +287,39 @@ __device__ __forceinline__ void fold_row(A4 &dst, const A4 &ghost) {
     dst.b += ghost.b;
 }
 
-template <int STAGE, int BC>
-struct AdjMarch {
+// A wave over a strip of 58 output columns (lanes 3..60) with a 3-column halo on each
+// side: lane l holds column j0 - 3 + l.  Loads mirror ghost rows and columns (with the
+// reflecting walls' sign flips) exactly as the forward stage kernel does.
+template <int BC>
+struct Lanes {
     const AdjArgs &A;
-    int s, ng, nx, ny, r0, r1, lane, j0;
-    unsigned j_bytes, jc_bytes;
+    int s, ng, nx, ny, lane, j0;
+    unsigned j_bytes, jc_bytes;  // own column / mirrored source column, in bytes
     bool gcol;   // the lane's column is outside the interior (its values are mirrored)
     bool owner;  // the lane writes its column
-    bool xact;   // x faces of this column are in the domain (interior column)
+    bool xact;   // interior column: x faces live, lamW nonzero
     bool yact;   // the y face on the lane's upper side (j | j+1) is in the domain
-    double half_g, g, kx, ky;  // kx = inv_dx * c_s * dt, ky = inv_dy * c_s * dt
-    bool split;
-    bool top;       // the tile starts at the first face (ng-1 | ng): it folds the top ghost rows
-    double *spill;  // this wave's LDS slot for the ghost row ng-2 (4 x 64 doubles)
+
+    __device__ __forceinline__ Lanes(const GridDev &G, const AdjArgs &A_, int strip, int lane_)
+        : A(A_), s(G.stride), ng(G.ng), nx(G.nx), ny(G.ny), lane(lane_) {
+        j0 = ng + strip * kAdjCols;
+        const int j = j0 - 3 + lane;
+        const int jcl = j < ng - 2 ? ng - 2 : (j > ng + ny + 1 ? ng + ny + 1 : j);
+        const int jc = mirror<BC>(jcl, ng, ng + ny, gcol);
+        j_bytes = (unsigned)(j < 0 ? 0 : j) * 8u;
+        jc_bytes = (unsigned)jc * 8u;
+        gcol = j < ng || j >= ng + ny;
+        owner = lane >= 3 && lane < 3 + kAdjCols && j < ng + ny;
+        xact = j >= ng && j < ng + ny;
+        yact = j >= ng - 1 && j <= ng + ny - 1;
+    }
 
     __device__ __forceinline__ long long row_off(int r) const {
         return (long long)__builtin_amdgcn_readfirstlane(r * s);
     }
 
-    __device__ __forceinline__ PCell load(int r) {
+    // Row r of V (r within 2 of the interior); `floor` = the stored cell is the floor state.
+    __device__ __forceinline__ PCell load(int r, bool &floor) const {
         bool gr;
         const int rs = mirror<BC>(r, ng, ng + nx, gr);
         const long long ro = row_off(rs);
@@ -305,15 +328,12 @@ struct AdjMarch {
         c.hu = ld(row_ptr(A.v_hu, ro), jc_bytes);
         c.hv = ld(row_ptr(A.v_hv, ro), jc_bytes);
         c.b = ld(row_ptr(A.B, ro), jc_bytes);
-        // the floor mask of V, for the adjoint of the stage that produced V
-        if (!gr && r >= r0 && r < r1 && owner)
-            A.mask_in[ro + (j_bytes >> 3)] = floor_state(h, c.hu, c.hv) ? 1 : 0;
+        floor = floor_state(h, c.hu, c.hv);
         if (BC == 0) {
             if (gr) c.hu = -c.hu;
             if (gcol) c.hv = -c.hv;
         }
         c.eta = h + c.b;
-        split = split && __all(split_ok(h) && split_ok(c.hu) && split_ok(c.hv) && split_ok(c.b));
         return c;
     }
 
@@ -335,9 +355,15 @@ struct AdjMarch {
         return l;
     }
 
-    // y faces of row q: adjoint contributions to the row's cells, ghost columns folded.
+    __device__ __forceinline__ static bool split_row(const PCell &c) {
+        return split_ok(c.eta - c.b) && split_ok(c.hu) && split_ok(c.hv) && split_ok(c.b);
+    }
+
+    // y faces of one row: the adjoint contributions to the row's cells, ghost columns
+    // folded into the columns they mirror.
     template <bool SPLIT>
-    __device__ __forceinline__ void y_row(const PCell &P, const D3 &L, A4 &acc) {
+    __device__ __forceinline__ A4 y_row(double half_g, double g, double ky, const PCell &P,
+                                        const D3 &L) const {
         const double de = from_next(P.eta) - P.eta;
         const double du = from_next(P.hu) - P.hu;
         const double dv = from_next(P.hv) - P.hv;
@@ -361,22 +387,23 @@ struct AdjMarch {
         // slope adjoints of this cell (the minus state uses +sigma, the plus state -sigma),
         // routed by the minmod selection: own part, lane j+1's "previous" part and lane
         // j-1's "next" part land here
+        A4 acc;
         double tp, ts, tn;
         slope_parts(se_, f.em - qe, tp, ts, tn);
-        acc.e += f.em + qe + ts + from_next(tp) + from_prev(tn);
+        acc.e = f.em + qe + ts + from_next(tp) + from_prev(tn);
         slope_parts(su_, f.tm - qt, tp, ts, tn);
-        acc.u += f.tm + qt + ts + from_next(tp) + from_prev(tn);
+        acc.u = f.tm + qt + ts + from_next(tp) + from_prev(tn);
         slope_parts(sv_, f.nm - qn, tp, ts, tn);
-        acc.v += f.nm + qn + ts + from_next(tp) + from_prev(tn);
-        acc.b += f.bm + qb;
+        acc.v = f.nm + qn + ts + from_next(tp) + from_prev(tn);
+        acc.b = f.bm + qb;
         fold_cols(acc);
+        return acc;
     }
 
     // Ghost columns (y walls) fold into the columns they mirror, hv negated for reflecting.
     __device__ __forceinline__ void fold_cols(A4 &acc) const {
-        // lane of column c is c - (j0 - 3)
-        const int base = j0 - 3;
-        if (j0 == ng) {  // left wall in this strip: ghosts ng-1, ng-2
+        const int base = j0 - 3;  // lane of column c is c - base
+        if (j0 == ng) {           // left wall in this strip: ghosts ng-1, ng-2
             fold_col(acc, ng - 1 - base, ng - base);
             fold_col(acc, ng - 2 - base, (BC == 0 ? ng + 1 : ng) - base);
         }
@@ -397,18 +424,207 @@ struct AdjMarch {
             acc.b += b;
         }
     }
+};
 
-    // One x face f | f+1 (plus the slope of row f+1 and the y faces of row f+1 when that
-    // row is written by this tile).  Window on entry: P0 = row f, P1 = row f+1, L0 = lamW of
-    // row f, S0/sel0 = slope of row f, Am/A0 = accumulators of rows f-1, f, pend = the
-    // slope adjoint of row f collected from face f-1, P2 = row f+2, L1 = lamW of row f+1.
-    // On exit: S1/sel1 = slope of row f+1, A1 = accumulator of row f+1, pend = the slope
-    // adjoint of row f+1 from face f.
-    template <bool SPLIT>
-    __device__ __forceinline__ void step_body(int f, const PCell &P0, const PCell &P1,
-                                              const PCell &P2, const D3 &L0, const D3 &L1,
-                                              const D3 &S0, int sel0, D3 &S1, int &sel1, A4 &Am,
-                                              A4 &A0, A4 &A1, D3 &pend) {
+template <int STAGE>
+__device__ __forceinline__ double stage_c() {
+    return STAGE == 1 ? 1.0 : (STAGE == 2 ? 0.25 : 2.0 / 3.0);
+}
+
+#ifndef SWE_ADJ_WAVES_PER_EU
+#define SWE_ADJ_WAVES_PER_EU 2
+#endif
+
+// ---- x gather ------------------------------------------------------------------------
+// Sums the x faces' adjoint parts of a row r (rows ng-2 .. ng+nx+1; faces outside the
+// domain contribute nothing): the direct parts of faces r (minus side) and r-1 (plus side)
+// and the slopes of rows r-1, r, r+1 routed by their minmod selections.
+template <int BC>
+struct XGather {
+    const GridDev &G;
+    const AdjArgs &A;
+    int j;
+    long long plane;
+
+    __device__ __forceinline__ double xf(int k, int f) const {
+        return A.xf[k * plane + (long long)(f - (G.ng - 1)) * G.stride + j];
+    }
+    __device__ __forceinline__ bool face_ok(int f) const {
+        return f >= G.ng - 1 && f <= G.ng + G.nx - 1;
+    }
+    __device__ __forceinline__ int sel(int t) const {
+        return A.xsel[(long long)(t - (G.ng - 1)) * G.stride + j];
+    }
+
+    __device__ __forceinline__ A4 row(int r) const {
+        A4 T{0.0, 0.0, 0.0, 0.0};
+        if (face_ok(r)) {
+            T.e += xf(0, r);
+            T.u += xf(1, r);
+            T.v += xf(2, r);
+            T.b += xf(3, r);
+        }
+        if (face_ok(r - 1)) {
+            T.e += xf(4, r - 1);
+            T.u += xf(5, r - 1);
+            T.v += xf(6, r - 1);
+            T.b += xf(7, r - 1);
+        }
+        for (int t = r - 1; t <= r + 1; ++t) {
+            if (t < G.ng - 1 || t > G.ng + G.nx) continue;
+            // slope adjoint of row t: +minus state of face t, -plus state of face t-1
+            double ge = 0.0, gu = 0.0, gv = 0.0;
+            if (face_ok(t)) {
+                ge += xf(0, t);
+                gu += xf(1, t);
+                gv += xf(2, t);
+            }
+            if (face_ok(t - 1)) {
+                ge -= xf(4, t - 1);
+                gu -= xf(5, t - 1);
+                gv -= xf(6, t - 1);
+            }
+            const int sl = sel(t);
+            double tp, ts, tn;
+            slope_parts(sl & 3, ge, tp, ts, tn);
+            T.e += t == r - 1 ? tn : (t == r ? ts : tp);
+            slope_parts((sl >> 2) & 3, gu, tp, ts, tn);
+            T.u += t == r - 1 ? tn : (t == r ? ts : tp);
+            slope_parts((sl >> 4) & 3, gv, tp, ts, tn);
+            T.v += t == r - 1 ? tn : (t == r ? ts : tp);
+        }
+        return T;
+    }
+};
+
+// ---- y faces, x gather and the stage's outputs: every row independently -----------------
+// Runs after adjoint_xface_kernel.  Per row: the y faces' adjoint (lanes, DPP), the x
+// faces' parts gathered from xf/xsel (ghost rows folded at the x walls), then the stage's
+// outputs: dL/dV (stage 1: plus the U^n accumulator), the U^n accumulator, dL/dB.  A wave
+// takes `y_rows` consecutive rows, issuing the next row's loads before each row's
+// arithmetic.
+template <int STAGE, int BC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SWE_ADJ_WAVES_PER_EU)))
+void adjoint_y_kernel(const GridDev G, const AdjArgs A) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int strip = blockIdx.x * kBlockWaves + threadIdx.x / kWave;
+    if (strip >= A.nstrips) return;
+    const Lanes<BC> w(G, A, strip, lane);
+    const int ng = G.ng, nx = G.nx;
+    const int r0 = ng + (int)blockIdx.y * A.y_rows;
+    const int r1 = (r0 + A.y_rows < ng + nx) ? r0 + A.y_rows : ng + nx;
+    constexpr double cv = STAGE == 1 ? 1.0 : (STAGE == 2 ? 0.25 : 2.0 / 3.0);
+    const double ky = G.inv_dy * (cv * *A.dt);
+    const XGather<BC> X{G, A, (int)(w.j_bytes >> 3), (long long)(nx + 1) * G.stride};
+    bool fl;
+    PCell P = w.load(r0, fl);
+    D3 L = w.load_lam(r0);
+#pragma unroll 1
+    for (int r = r0; r < r1; ++r) {
+        const int rn = r + 1 < r1 ? r + 1 : r;
+        const PCell Pn = w.load(rn, fl);
+        const D3 Ln = w.load_lam(rn);
+        A4 acc = (G.split_ok && __all(Lanes<BC>::split_row(P)))
+                     ? w.template y_row<true>(G.half_g, G.g, ky, P, L)
+                     : w.template y_row<false>(G.half_g, G.g, ky, P, L);
+        if (w.owner) {
+            A4 T = X.row(r);
+            if (r == ng) {
+                fold_row<BC>(T, X.row(ng - 1));
+                if (BC != 0) fold_row<BC>(T, X.row(ng - 2));
+            }
+            if (BC == 0 && r == ng + 1) fold_row<BC>(T, X.row(ng - 2));
+            if (r == ng + nx - 1) {
+                fold_row<BC>(T, X.row(ng + nx));
+                if (BC != 0) fold_row<BC>(T, X.row(ng + nx + 1));
+            }
+            if (BC == 0 && r == ng + nx - 2) fold_row<BC>(T, X.row(ng + nx + 1));
+            const long long ro = w.row_off(r);
+            const unsigned jb = w.j_bytes;
+            double oh = cv * L.e + (T.e + acc.e);
+            double ou = cv * L.u + (T.u + acc.u);
+            double ov = cv * L.v + (T.v + acc.v);
+            if constexpr (STAGE == 3) {
+                st(row_ptr(A.e_h, ro), jb, (1.0 / 3.0) * L.e);
+                st(row_ptr(A.e_hu, ro), jb, (1.0 / 3.0) * L.u);
+                st(row_ptr(A.e_hv, ro), jb, (1.0 / 3.0) * L.v);
+            } else if constexpr (STAGE == 2) {
+                st(row_ptr(A.e_h, ro), jb, ld(row_ptr(A.e_h, ro), jb) + 0.75 * L.e);
+                st(row_ptr(A.e_hu, ro), jb, ld(row_ptr(A.e_hu, ro), jb) + 0.75 * L.u);
+                st(row_ptr(A.e_hv, ro), jb, ld(row_ptr(A.e_hv, ro), jb) + 0.75 * L.v);
+            } else {
+                oh += ld(row_ptr(A.e_h, ro), jb);
+                ou += ld(row_ptr(A.e_hu, ro), jb);
+                ov += ld(row_ptr(A.e_hv, ro), jb);
+            }
+            st(row_ptr(A.out_h, ro), jb, oh);
+            st(row_ptr(A.out_hu, ro), jb, ou);
+            st(row_ptr(A.out_hv, ro), jb, ov);
+            st(row_ptr(A.lam_b, ro), jb,
+               ld(row_ptr(A.lam_b, ro), jb) + ((T.e + T.b) + (acc.e + acc.b)));
+        }
+        P = Pn;
+        L = Ln;
+    }
+}
+
+// ---- x faces: reverse each face f | f+1 independently -------------------------------------
+// Faces f = ng-1 .. ng+nx-1 (index i = f - (ng-1)).  A wave takes `tile_rows` consecutive
+// faces of its strip, carrying only the primal rows (each row loaded once per wave) and
+// the slopes.  Per face it writes the 8 adjoint outputs (eta, hu, hv, bottom of the minus
+// and plus states) to xf[k][i][j], and per row q = f the minmod selections of its slope to
+// sel[q - (ng-1)][j] (plus row ng+nx from the last face).  It also writes the floor mask
+// of V for the rows it owns (q = f, interior).
+template <int STAGE, int BC>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SWE_ADJ_WAVES_PER_EU)))
+void adjoint_xface_kernel(const GridDev G, const AdjArgs A) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int strip = blockIdx.x * kBlockWaves + threadIdx.x / kWave;
+    if (strip >= A.nstrips) return;
+    const Lanes<BC> w(G, A, strip, lane);
+    const int ng = G.ng, nx = G.nx;
+    const int f0 = ng - 1 + (int)blockIdx.y * A.tile_rows;
+    const int f1 = (f0 + A.tile_rows < ng + nx) ? f0 + A.tile_rows : ng + nx;
+    const double kx = G.inv_dx * (stage_c<STAGE>() * *A.dt);
+    const long long plane = (long long)(nx + 1) * G.stride;  // one component of xf
+    bool split = G.split_ok != 0;
+    bool fl;
+    // floor mask of V for the rows this wave owns (q = f for its faces, interior q)
+    auto mark = [&](int r, bool floor) {
+        if (r >= f0 && r < f1 && r >= ng && r < ng + nx && w.owner)
+            A.mask_in[w.row_off(r) + (w.j_bytes >> 3)] = floor ? 1 : 0;
+    };
+
+    PCell P0, P1, P2, P3;
+    D3 S0, S1;
+    int sel0, sel1;
+    {
+        const PCell Pm = w.load(f0 - 1, fl);
+        split = split && __all(Lanes<BC>::split_row(Pm));
+        P0 = w.load(f0, fl);
+        mark(f0, fl);
+        P1 = w.load(f0 + 1, fl);
+        mark(f0 + 1, fl);
+        P2 = w.load(f0 + 2, fl);
+        mark(f0 + 2, fl);
+        split = split && __all(Lanes<BC>::split_row(P0) && Lanes<BC>::split_row(P1) &&
+                               Lanes<BC>::split_row(P2));
+        int sa, sb, sc;
+        S0.e = half_minmod(P0.eta - Pm.eta, P1.eta - P0.eta, sa);
+        S0.u = half_minmod(P0.hu - Pm.hu, P1.hu - P0.hu, sb);
+        S0.v = half_minmod(P0.hv - Pm.hv, P1.hv - P0.hv, sc);
+        sel0 = sa | (sb << 2) | (sc << 4);
+    }
+    D3 L0 = w.load_lam(f0), L1 = w.load_lam(f0 + 1), L2;
+
+#pragma unroll 1
+    for (int f = f0; f < f1; ++f) {
+        // next face's row of V (written to the floor mask when owned) and lamW
+        const int rn = f + 3 <= ng + nx + 1 ? f + 3 : ng + nx + 1;
+        P3 = w.load(rn, fl);
+        if (rn == f + 3) mark(rn, fl);
+        L2 = w.load_lam(f + 2);
+
         int sa, sb, sc;
         S1.e = half_minmod(P1.eta - P0.eta, P2.eta - P1.eta, sa);
         S1.u = half_minmod(P1.hu - P0.hu, P2.hu - P1.hu, sb);
@@ -416,193 +632,40 @@ struct AdjMarch {
         sel1 = sa | (sb << 2) | (sc << 4);
 
         // x faces: normal momentum hu, tangential hv
-        FaceAdj x = face_adjoint<SPLIT>(half_g, g, P0.eta + S0.e, P0.hu + S0.u, P0.hv + S0.v,
-                                        P0.b, P1.eta - S1.e, P1.hu - S1.u, P1.hv - S1.v, P1.b,
-                                        -kx * L0.e, -kx * L0.u, -kx * L0.v, kx * L1.e,
-                                        kx * L1.u, kx * L1.v);
-        if (!xact) x = face_zero();
-        A0.e += x.em;
-        A0.u += x.nm;
-        A0.v += x.tm;
-        A0.b += x.bm;
-        A1 = A4{x.ep, x.np, x.tp, x.bp};
-        // slope of row f is complete: route it to rows f-1, f, f+1
-        const D3 g0{pend.e + x.em, pend.u + x.nm, pend.v + x.tm};
-        pend = D3{-x.ep, -x.np, -x.tp};
-        double tp, ts, tn;
-        slope_parts(sel0 & 3, g0.e, tp, ts, tn);
-        Am.e += tp;
-        A0.e += ts;
-        A1.e += tn;
-        slope_parts((sel0 >> 2) & 3, g0.u, tp, ts, tn);
-        Am.u += tp;
-        A0.u += ts;
-        A1.u += tn;
-        slope_parts((sel0 >> 4) & 3, g0.v, tp, ts, tn);
-        Am.v += tp;
-        A0.v += ts;
-        A1.v += tn;
-
-        if (f + 1 >= r0 && f + 1 < r1) y_row<SPLIT>(P1, L1, A1);
-    }
-
-    // Loads of the next step (row f+3 of V, lamW of row f+2) are issued before this step's
-    // arithmetic, so their latency hides behind one step of VALU work.
-    __device__ __forceinline__ void step(int f, const PCell &P0, const PCell &P1,
-                                         const PCell &P2, PCell &P3, const D3 &L0, const D3 &L1,
-                                         D3 &L2, const D3 &S0, int sel0, D3 &S1, int &sel1,
-                                         A4 &Am, A4 &A0, A4 &A1, D3 &pend) {
-        P3 = load(f + 3 <= ng + nx + 1 ? f + 3 : ng + nx + 1);
-        L2 = load_lam(f + 2);
+        FaceAdj x;
         if (split)
-            step_body<true>(f, P0, P1, P2, L0, L1, S0, sel0, S1, sel1, Am, A0, A1, pend);
+            x = face_adjoint<true>(G.half_g, G.g, P0.eta + S0.e, P0.hu + S0.u, P0.hv + S0.v,
+                                   P0.b, P1.eta - S1.e, P1.hu - S1.u, P1.hv - S1.v, P1.b,
+                                   -kx * L0.e, -kx * L0.u, -kx * L0.v, kx * L1.e, kx * L1.u,
+                                   kx * L1.v);
         else
-            step_body<false>(f, P0, P1, P2, L0, L1, S0, sel0, S1, sel1, Am, A0, A1, pend);
-    }
-
-    // Row q is complete: write it (interior rows of this tile), or fold it (ghost rows).
-    __device__ __forceinline__ void retire(int q, const A4 &Aq, A4 &A0, A4 &A1) {
-        if (top && q == ng - 2) {  // ghost row mirrored onto ng+1 (reflecting) / ng: not yet open
-            spill[0 * kWave + lane] = Aq.e;
-            spill[1 * kWave + lane] = Aq.u;
-            spill[2 * kWave + lane] = Aq.v;
-            spill[3 * kWave + lane] = Aq.b;
-            return;
+            x = face_adjoint<false>(G.half_g, G.g, P0.eta + S0.e, P0.hu + S0.u, P0.hv + S0.v,
+                                    P0.b, P1.eta - S1.e, P1.hu - S1.u, P1.hv - S1.v, P1.b,
+                                    -kx * L0.e, -kx * L0.u, -kx * L0.v, kx * L1.e, kx * L1.u,
+                                    kx * L1.v);
+        if (w.owner) {
+            const long long fo = (long long)__builtin_amdgcn_readfirstlane((f - (ng - 1)) * w.s);
+            double *xf = A.xf + fo;
+            st(row_ptr(xf, 0 * plane), w.j_bytes, x.em);
+            st(row_ptr(xf, 1 * plane), w.j_bytes, x.nm);
+            st(row_ptr(xf, 2 * plane), w.j_bytes, x.tm);
+            st(row_ptr(xf, 3 * plane), w.j_bytes, x.bm);
+            st(row_ptr(xf, 4 * plane), w.j_bytes, x.ep);
+            st(row_ptr(xf, 5 * plane), w.j_bytes, x.np);
+            st(row_ptr(xf, 6 * plane), w.j_bytes, x.tp);
+            st(row_ptr(xf, 7 * plane), w.j_bytes, x.bp);
+            unsigned char *sr = A.xsel + fo;
+            sr[w.j_bytes >> 3] = (unsigned char)sel0;
+            if (f == ng + nx - 1) sr[w.s + (w.j_bytes >> 3)] = (unsigned char)sel1;
         }
-        if (top && q == ng - 1) {  // rows ng (A0) and ng+1 (A1) are open
-            fold_row<BC>(A0, Aq);
-            const A4 g2{spill[0 * kWave + lane], spill[1 * kWave + lane],
-                        spill[2 * kWave + lane], spill[3 * kWave + lane]};
-            fold_row<BC>(BC == 0 ? A1 : A0, g2);
-            return;
-        }
-        if (q < r0 || q >= r1) return;
-        emit(q, Aq);
-    }
-
-    __device__ __forceinline__ void emit(int q, const A4 &Aq) const {
-        if (!owner) return;
-        const D3 Lq = load_lam(q);  // reloaded (a cache hit) rather than kept in registers
-        const long long ro = row_off(q);
-        constexpr double cv = STAGE == 1 ? 1.0 : (STAGE == 2 ? 0.25 : 2.0 / 3.0);
-        double oh = cv * Lq.e + Aq.e;
-        double ou = cv * Lq.u + Aq.u;
-        double ov = cv * Lq.v + Aq.v;
-        if constexpr (STAGE == 3) {
-            st(row_ptr(A.e_h, ro), j_bytes, (1.0 / 3.0) * Lq.e);
-            st(row_ptr(A.e_hu, ro), j_bytes, (1.0 / 3.0) * Lq.u);
-            st(row_ptr(A.e_hv, ro), j_bytes, (1.0 / 3.0) * Lq.v);
-        } else if constexpr (STAGE == 2) {
-            st(row_ptr(A.e_h, ro), j_bytes, ld(row_ptr(A.e_h, ro), j_bytes) + 0.75 * Lq.e);
-            st(row_ptr(A.e_hu, ro), j_bytes, ld(row_ptr(A.e_hu, ro), j_bytes) + 0.75 * Lq.u);
-            st(row_ptr(A.e_hv, ro), j_bytes, ld(row_ptr(A.e_hv, ro), j_bytes) + 0.75 * Lq.v);
-        } else {
-            oh += ld(row_ptr(A.e_h, ro), j_bytes);
-            ou += ld(row_ptr(A.e_hu, ro), j_bytes);
-            ov += ld(row_ptr(A.e_hv, ro), j_bytes);
-        }
-        st(row_ptr(A.out_h, ro), j_bytes, oh);
-        st(row_ptr(A.out_hu, ro), j_bytes, ou);
-        st(row_ptr(A.out_hv, ro), j_bytes, ov);
-        st(row_ptr(A.lam_b, ro), j_bytes, ld(row_ptr(A.lam_b, ro), j_bytes) + (Aq.e + Aq.b));
-    }
-};
-
-#ifndef SWE_ADJ_WAVES_PER_EU
-#define SWE_ADJ_WAVES_PER_EU 1
-#endif
-
-template <int STAGE, int BC>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SWE_ADJ_WAVES_PER_EU)))
-void adjoint_stage_kernel(const GridDev G, const AdjArgs A) {
-    __shared__ double spill_lds[kBlockWaves][4 * kWave];
-    const int lane = threadIdx.x & (kWave - 1);
-    const int wave = threadIdx.x / kWave;
-    const int strip = blockIdx.x * kBlockWaves + wave;
-    if (strip >= A.nstrips) return;
-
-    const int ng = G.ng, nx = G.nx, ny = G.ny;
-    const int j0 = ng + strip * kAdjCols;
-    const int j = j0 - 3 + lane;
-    const int jcl = j < ng - 2 ? ng - 2 : (j > ng + ny + 1 ? ng + ny + 1 : j);
-    bool gcol;
-    const int jc = mirror<BC>(jcl, ng, ng + ny, gcol);
-    const int r0 = ng + (int)blockIdx.y * A.tile_rows;
-    const int r1 = (r0 + A.tile_rows < ng + nx) ? r0 + A.tile_rows : ng + nx;
-    const double dt = *A.dt;
-    constexpr double cs = STAGE == 1 ? 1.0 : (STAGE == 2 ? 0.25 : 2.0 / 3.0);
-
-    AdjMarch<STAGE, BC> m{A, G.stride, ng, nx, ny, r0, r1, lane, j0,
-                          (unsigned)(j < 0 ? 0 : j) * 8u, (unsigned)jc * 8u,
-                          j < ng || j >= ng + ny,
-                          lane >= 3 && lane < 3 + kAdjCols && j < ng + ny,
-                          j >= ng && j < ng + ny,
-                          j >= ng - 1 && j <= ng + ny - 1,
-                          G.half_g, G.g, G.inv_dx * (cs * dt), G.inv_dy * (cs * dt),
-                          G.split_ok != 0, r0 - 2 <= ng - 1, &spill_lds[wave][0]};
-    (void)gcol;
-
-    const int fa = (r0 - 2 > ng - 1) ? r0 - 2 : ng - 1;
-    const int fb = (r1 < ng + nx - 1) ? r1 : ng + nx - 1;
-    const bool bottom = fb == ng + nx - 1;
-
-    // prologue: rows fa-1 .. fa+2, slope of row fa, lamW of rows fa, fa+1
-    PCell P0, P1, P2, P3;
-    D3 S0, S1;
-    int sel0, sel1;
-    {
-        const PCell Pm = m.load(fa - 1);
-        P0 = m.load(fa);
-        P1 = m.load(fa + 1);
-        P2 = m.load(fa + 2);
-        int sa, sb, sc;
-        S0.e = half_minmod(P0.eta - Pm.eta, P1.eta - P0.eta, sa);
-        S0.u = half_minmod(P0.hu - Pm.hu, P1.hu - P0.hu, sb);
-        S0.v = half_minmod(P0.hv - Pm.hv, P1.hv - P0.hv, sc);
-        sel0 = sa | (sb << 2) | (sc << 4);
-    }
-    D3 L0 = m.load_lam(fa), L1 = m.load_lam(fa + 1), L2;
-    A4 Am{0.0, 0.0, 0.0, 0.0}, A0{0.0, 0.0, 0.0, 0.0}, A1;
-    D3 pend{0.0, 0.0, 0.0};
-
-#pragma unroll 1
-    for (int f = fa; f <= fb; ++f) {
-        m.step(f, P0, P1, P2, P3, L0, L1, L2, S0, sel0, S1, sel1, Am, A0, A1, pend);
-        if (f == fb && bottom) break;
-        m.retire(f - 1, Am, A0, A1);
+        split = split && __all(Lanes<BC>::split_row(P3));
         P0 = P1;
         P1 = P2;
         P2 = P3;
-        L0 = L1;
-        L1 = L2;
         S0 = S1;
         sel0 = sel1;
-        Am = A0;
-        A0 = A1;
-    }
-    if (bottom) {
-        // After the last face (ng+nx-1 | ng+nx): Am = row ng+nx-2, A0 = row ng+nx-1,
-        // A1 = ghost row ng+nx; the slope of ghost row ng+nx (S1/sel1, adjoint `pend`) is
-        // complete and reaches ghost row ng+nx+1 too.
-        A4 A2{0.0, 0.0, 0.0, 0.0};
-        double tp, ts, tn;
-        slope_parts(sel1 & 3, pend.e, tp, ts, tn);
-        A0.e += tp;
-        A1.e += ts;
-        A2.e += tn;
-        slope_parts((sel1 >> 2) & 3, pend.u, tp, ts, tn);
-        A0.u += tp;
-        A1.u += ts;
-        A2.u += tn;
-        slope_parts((sel1 >> 4) & 3, pend.v, tp, ts, tn);
-        A0.v += tp;
-        A1.v += ts;
-        A2.v += tn;
-        fold_row<BC>(A0, A1);                   // ng+nx   -> ng+nx-1
-        fold_row<BC>(BC == 0 ? Am : A0, A2);    // ng+nx+1 -> ng+nx-2 (reflecting) / ng+nx-1
-        const int q = ng + nx - 2;
-        if (q >= r0 && q < r1) m.emit(q, Am);
-        if (q + 1 >= r0 && q + 1 < r1) m.emit(q + 1, A0);
+        L0 = L1;
+        L1 = L2;
     }
 }
 
@@ -641,19 +704,26 @@ __global__ __launch_bounds__(256) void zero_ghosts_kernel(const GridDev G, doubl
 }
 
 template <int STAGE, int BC>
-hipError_t launch_adj_one(const GridDev &G, const AdjArgs &A, hipStream_t stream, dim3 grid) {
-    hipLaunchKernelGGL((adjoint_stage_kernel<STAGE, BC>), grid, dim3(kWave * kBlockWaves), 0,
+hipError_t launch_adj_one(const GridDev &G, const AdjArgs &A, hipStream_t stream) {
+    const dim3 block(kWave * kBlockWaves);
+    const unsigned sx = (unsigned)((A.nstrips + kBlockWaves - 1) / kBlockWaves);
+    hipLaunchKernelGGL((adjoint_xface_kernel<STAGE, BC>),
+                       dim3(sx, (unsigned)((G.nx + 1 + A.tile_rows - 1) / A.tile_rows)), block, 0,
                        stream, G, A);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((adjoint_y_kernel<STAGE, BC>),
+                       dim3(sx, (unsigned)((G.nx + A.y_rows - 1) / A.y_rows)), block, 0, stream,
+                       G, A);
     return hipGetLastError();
 }
 
 template <int BC>
-hipError_t launch_adj_bc(int stage, const GridDev &G, const AdjArgs &A, hipStream_t stream,
-                         dim3 grid) {
+hipError_t launch_adj_bc(int stage, const GridDev &G, const AdjArgs &A, hipStream_t stream) {
     switch (stage) {
-        case 1: return launch_adj_one<1, BC>(G, A, stream, grid);
-        case 2: return launch_adj_one<2, BC>(G, A, stream, grid);
-        case 3: return launch_adj_one<3, BC>(G, A, stream, grid);
+        case 1: return launch_adj_one<1, BC>(G, A, stream);
+        case 2: return launch_adj_one<2, BC>(G, A, stream);
+        case 3: return launch_adj_one<3, BC>(G, A, stream);
         default: return hipErrorInvalidValue;
     }
 }
@@ -664,12 +734,10 @@ int adj_strips_for(int ny) { return (ny + kAdjCols - 1) / kAdjCols; }
 
 hipError_t launch_adjoint_stage(int stage, int bc, const GridDev &G, const AdjArgs &A,
                                 hipStream_t stream) {
-    if (G.ng != 2 || A.tile_rows < 1 || A.nstrips != adj_strips_for(G.ny))
+    if (G.ng != 2 || A.tile_rows < 1 || A.y_rows < 1 || A.nstrips != adj_strips_for(G.ny) ||
+        A.xf == nullptr || A.xsel == nullptr)
         return hipErrorInvalidValue;
-    const dim3 grid((unsigned)((A.nstrips + kBlockWaves - 1) / kBlockWaves),
-                    (unsigned)((G.nx + A.tile_rows - 1) / A.tile_rows), 1);
-    return bc == 0 ? launch_adj_bc<0>(stage, G, A, stream, grid)
-                   : launch_adj_bc<1>(stage, G, A, stream, grid);
+    return bc == 0 ? launch_adj_bc<0>(stage, G, A, stream) : launch_adj_bc<1>(stage, G, A, stream);
 }
 
 hipError_t launch_floor_mask(const GridDev &G, const double *h, const double *hu,

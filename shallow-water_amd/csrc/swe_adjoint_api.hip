@@ -97,17 +97,22 @@ struct swe_hip_adj {
     int64_t nsteps = 0;
     int32_t nsnap = 0;
     size_t n_total = 0;
-    double *slots = nullptr;  // nsnap states of 3 arrays
+    double *slots = nullptr;  // nsnap states of 3 arrays (9 with stage records)
+    bool records = false;     // slot p also holds U1, U2 of step p (store-all only)
+    int per_slot = 3;
     double *W[3] = {nullptr, nullptr, nullptr};
     double *lam[4][3] = {};   // step adjoint, dL/dU2, dL/dU1, U^n accumulator
     double *lam_b = nullptr;
     unsigned char *mask[2] = {nullptr, nullptr};
     double *dts = nullptr;
     double *partial = nullptr;  // 1024 per-block maxima (adaptive dt)
+    double *xf = nullptr;       // adjoint outputs of the x faces (8 per face and column)
+    unsigned char *xsel = nullptr;  // minmod selections of the x slopes
     long long *counter = nullptr;
     std::vector<Action> acts;
     size_t first_adj = 0;
-    int tile_rows = 16;
+    int tile_rows = 4;   // x-face kernel faces per wave
+    int y_rows = 4;      // y-face kernel rows per wave (no halo cost: short is fine)
     int state = 0;  // 0 created, 1 forward done (checkpoints valid), 2 consumed
     int parity = 0;
     double fwd_ms = 0.0, bwd_ms = 0.0;
@@ -129,7 +134,7 @@ int afail(swe_hip_adj *a, int code, const std::string &msg) {
     } while (0)
 
 double *slot_ptr(swe_hip_adj *a, int slot, int comp) {
-    return a->slots + ((size_t)slot * 3 + comp) * a->n_total;
+    return a->slots + ((size_t)slot * a->per_slot + comp) * a->n_total;
 }
 
 struct State {
@@ -163,10 +168,21 @@ swe::StageArgs fwd_args(swe_hip_adj *a, const State &in, const State &u0, const 
     return s;
 }
 
-// Stages 1 and 2 of step p from `base` into the context's U1, U2.
+// Where stages 1 and 2 of step p go: the step's record (stage records: slot p holds U^p,
+// U1, U2) or the context's stage buffers.
+State stage_state(swe_hip_adj *a, int64_t p, int which) {
+    swe_hip_ctx *c = a->ctx;
+    if (a->records)
+        return State{{slot_ptr(a, (int)p, 3 * which), slot_ptr(a, (int)p, 3 * which + 1),
+                      slot_ptr(a, (int)p, 3 * which + 2)}};
+    double *const *b = which == 1 ? c->U1 : c->U2;
+    return State{{b[0], b[1], b[2]}};
+}
+
+// Stages 1 and 2 of step p from `base`.
 int recompute_u1_u2(swe_hip_adj *a, const State &base, int64_t p) {
     swe_hip_ctx *c = a->ctx;
-    const State u1{{c->U1[0], c->U1[1], c->U1[2]}}, u2{{c->U2[0], c->U2[1], c->U2[2]}};
+    const State u1 = stage_state(a, p, 1), u2 = stage_state(a, p, 2);
     ADJ_CHECK(a, swe::launch_stage(1, c->bc, false, false, c->G,
                                    fwd_args(a, base, base, u1, a->dts + p), c->stream, nullptr));
     ADJ_CHECK(a, swe::launch_stage(2, c->bc, false, false, c->G,
@@ -186,7 +202,7 @@ int forward_step(swe_hip_adj *a, const State &in, const State &out, int64_t p, d
     }
     const int rc = recompute_u1_u2(a, in, p);
     if (rc) return rc;
-    const State u2{{c->U2[0], c->U2[1], c->U2[2]}};
+    const State u2 = stage_state(a, p, 2);
     ADJ_CHECK(a, swe::launch_stage(3, c->bc, false, false, c->G,
                                    fwd_args(a, u2, in, out, a->dts + p), c->stream, nullptr));
     return SWE_OK;
@@ -225,7 +241,10 @@ swe::AdjArgs adj_args(swe_hip_adj *a, const State &v, int lam_in, int lam_out,
     r.mask_in = mask_in;
     r.dt = a->dts + p;
     r.tile_rows = a->tile_rows;
+    r.y_rows = a->y_rows;
     r.nstrips = swe::adj_strips_for(c->G.ny);
+    r.xf = a->xf;
+    r.xsel = a->xsel;
     return r;
 }
 
@@ -233,9 +252,11 @@ swe::AdjArgs adj_args(swe_hip_adj *a, const State &v, int lam_in, int lam_out,
 int reverse_step(swe_hip_adj *a, int src, int64_t p) {
     swe_hip_ctx *c = a->ctx;
     const State base = state_of(a, src);
-    int rc = recompute_u1_u2(a, base, p);
-    if (rc) return rc;
-    const State u1{{c->U1[0], c->U1[1], c->U1[2]}}, u2{{c->U2[0], c->U2[1], c->U2[2]}};
+    if (!a->records) {  // with stage records the forward pass left U1, U2 of step p in place
+        const int rc = recompute_u1_u2(a, base, p);
+        if (rc) return rc;
+    }
+    const State u1 = stage_state(a, p, 1), u2 = stage_state(a, p, 2);
     unsigned char *m0 = a->mask[a->parity], *m1 = a->mask[1 - a->parity];
     // stage 3: V = U2, output U^{p+1} (mask m0)
     ADJ_CHECK(a, swe::launch_adjoint_stage(3, c->bc, c->G, adj_args(a, u2, 0, 1, m0, m1, p),
@@ -267,6 +288,7 @@ int backward_common(swe_hip_adj *a, const double *lh, const double *lhu, const d
         return afail(a, SWE_ERR_INVALID, "backward needs a forward pass with fresh checkpoints");
     swe_hip_ctx *c = a->ctx;
     ADJ_CHECK(a, hipSetDevice(c->device));
+
     const auto t0 = std::chrono::steady_clock::now();
     const size_t bytes = a->n_total * sizeof(double);
     const double *lin[3] = {lh, lhu, lhv};
@@ -315,7 +337,7 @@ int swe_hip_adj_schedule(int64_t nsteps, int32_t nsnap, int64_t *advances, int32
 }
 
 int swe_hip_adj_create(swe_hip_ctx *ctx, int64_t nsteps, int32_t nsnap, swe_hip_adj **out) {
-    if (!ctx || !out || nsteps < 1 || nsnap < 0) return SWE_ERR_INVALID;
+    if (!ctx || !out || nsteps < 1 || nsnap < -1) return SWE_ERR_INVALID;
     *out = nullptr;
     if (ctx->G.ng != 2) return SWE_ERR_INVALID;
     if (hipSetDevice(ctx->device) != hipSuccess) return SWE_ERR_HIP;
@@ -336,27 +358,44 @@ int swe_hip_adj_create(swe_hip_ctx *ctx, int64_t nsteps, int32_t nsnap, swe_hip_
     if (hipMalloc(&a->lam_b, bytes) != hipSuccess) return bad(SWE_ERR_OOM);
     for (auto &m : a->mask)
         if (hipMalloc(&m, a->n_total) != hipSuccess) return bad(SWE_ERR_OOM);
+    const size_t xfaces = (size_t)(ctx->G.nx + 1) * ctx->G.stride;
+    if (hipMalloc(&a->xf, 8 * xfaces * sizeof(double)) != hipSuccess ||
+        hipMalloc(&a->xsel, xfaces + ctx->G.stride) != hipSuccess)
+        return bad(SWE_ERR_OOM);
     if (hipMalloc(&a->dts, sizeof(double) * (size_t)nsteps) != hipSuccess ||
         hipMalloc(&a->partial, sizeof(double) * 1024) != hipSuccess ||
         hipMalloc(&a->counter, sizeof(long long)) != hipSuccess)
         return bad(SWE_ERR_OOM);
     const size_t state_bytes = 3 * bytes;
     if (nsnap == 0) {
+        // automatic: every step with its stage states if that fits in 60% of the free
+        // memory, else as many stored states as half of it holds
         size_t free_b = 0, total_b = 0;
         if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return bad(SWE_ERR_HIP);
-        const int64_t fit = (int64_t)(free_b / 2 / state_bytes);
-        nsnap = (int32_t)std::max<int64_t>(1, std::min<int64_t>({nsteps, fit, INT32_MAX}));
+        if ((double)nsteps * 3.0 * (double)state_bytes <= 0.6 * (double)free_b &&
+            nsteps <= INT32_MAX) {
+            nsnap = -1;
+        } else {
+            const int64_t fit = (int64_t)(free_b / 2 / state_bytes);
+            nsnap = (int32_t)std::max<int64_t>(1, std::min<int64_t>({nsteps, fit, INT32_MAX}));
+        }
+    }
+    if (nsnap == -1) {
+        if (nsteps > INT32_MAX) return bad(SWE_ERR_INVALID);
+        nsnap = (int32_t)nsteps;
+        a->records = true;
+        a->per_slot = 9;
     }
     a->nsnap = nsnap;
-    if (hipMalloc(&a->slots, state_bytes * (size_t)nsnap) != hipSuccess) return bad(SWE_ERR_OOM);
+    if (hipMalloc(&a->slots, state_bytes * (size_t)(a->per_slot / 3) * (size_t)nsnap) !=
+        hipSuccess)
+        return bad(SWE_ERR_OOM);
     a->acts = make_schedule(nsteps, nsnap);
     for (size_t i = 0; i < a->acts.size(); ++i)
         if (a->acts[i].kind == kAdjoint) {
             a->first_adj = i;
             break;
         }
-    // taller tiles than the forward's: each tile re-reverses three halo faces
-    a->tile_rows = ctx->G.nx >= 512 ? 16 : 8;
     *out = a;
     return SWE_OK;
 }
@@ -374,6 +413,8 @@ void swe_hip_adj_destroy(swe_hip_adj *a) {
     (void)hipFree(a->dts);
     (void)hipFree(a->partial);
     (void)hipFree(a->counter);
+    (void)hipFree(a->xf);
+    (void)hipFree(a->xsel);
     delete a;
 }
 
@@ -441,16 +482,18 @@ int swe_hip_adj_backward_device(swe_hip_adj *a, const double *lh, const double *
                            hipMemcpyDeviceToDevice);
 }
 
-int swe_hip_adj_set_tile_rows(swe_hip_adj *a, int tile_rows) {
-    if (!a || tile_rows < 1) return SWE_ERR_INVALID;
-    a->tile_rows = tile_rows;
+int swe_hip_adj_set_launch(swe_hip_adj *a, int x_rows, int y_rows) {
+    if (!a || x_rows < 0 || y_rows < 0) return SWE_ERR_INVALID;
+    if (x_rows) a->tile_rows = x_rows;
+    if (y_rows) a->y_rows = y_rows;
     return SWE_OK;
 }
 
-int swe_hip_adj_info(const swe_hip_adj *a, int32_t *nsnap, double *forward_ms,
+int swe_hip_adj_info(const swe_hip_adj *a, int32_t *nsnap, int32_t *records, double *forward_ms,
                      double *backward_ms) {
     if (!a) return SWE_ERR_INVALID;
     if (nsnap) *nsnap = a->nsnap;
+    if (records) *records = a->records ? 1 : 0;
     if (forward_ms) *forward_ms = a->fwd_ms;
     if (backward_ms) *backward_ms = a->bwd_ms;
     return SWE_OK;

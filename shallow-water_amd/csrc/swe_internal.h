@@ -122,8 +122,11 @@ struct AdjArgs {
     double *lam_b;
     unsigned char *mask_in;
     const double *dt;  // device scalar: this step's dt (frozen)
-    int tile_rows;     // interior rows per wave
+    int tile_rows;     // x faces per wave of the x-face kernel
+    int y_rows;        // rows per wave of the y-face kernel
     int nstrips;       // 58-column strips across y (adj_strips_for)
+    double *xf;        // 8 x (nx+1) x stride: adjoint outputs of every x face
+    unsigned char *xsel;  // (nx+2) x stride: minmod selections of rows ng-1 .. ng+nx
 };
 
 int adj_strips_for(int ny);

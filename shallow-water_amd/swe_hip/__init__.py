@@ -102,10 +102,11 @@ _SIGS = {
     "swe_hip_adj_forward": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, _dp]),
     "swe_hip_adj_backward": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "swe_hip_adj_backward_device": (C.c_int, [C.c_void_p] + [C.c_void_p] * 7),
-    "swe_hip_adj_set_tile_rows": (C.c_int, [C.c_void_p, C.c_int]),
+    "swe_hip_adj_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "swe_hip_adj_schedule": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_int64),
                                        C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
-    "swe_hip_adj_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), _dp, _dp]),
+    "swe_hip_adj_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
+                                   _dp, _dp]),
 }
 
 _lib = None
@@ -380,8 +381,9 @@ class HipAdjoint:
     """Discrete adjoint of `nsteps` SSP-RK3 steps of a HipSolver (swe_hip_adj_* in the C ABI).
 
     forward() runs the steps from the solver's current state, storing binomial checkpoints
-    (nsnap stored states, U^0 included; 0 = as many as half the free device memory holds,
-    at most nsteps) and leaves U^N in the solver; backward(lam) takes dL/dU^N and returns
+    (nsnap stored states, U^0 included; -1 = every step with its stage states; 0 = -1 when
+    that fits in 60% of the free device memory, else as many states as half of it holds)
+    and leaves U^N in the solver; backward(lam) takes dL/dU^N and returns
     dL/dU^0 and dL/dB with dt frozen from the forward pass.  One backward per forward."""
 
     def __init__(self, solver: HipSolver, nsteps: int, nsnap: int = 0):
@@ -444,10 +446,13 @@ class HipAdjoint:
             self._a, lam_h.data_ptr(), lam_hu.data_ptr(), lam_hv.data_ptr(), g_h.data_ptr(),
             g_hu.data_ptr(), g_hv.data_ptr(), g_b.data_ptr() if g_b is not None else None))
 
-    def set_tile_rows(self, rows: int):
-        self._check(lib().swe_hip_adj_set_tile_rows(self._a, int(rows)))
+    def set_launch(self, x_rows: int = 0, y_rows: int = 0):
+        """Rows per wave of the x-march and y-face adjoint kernels (0 keeps the value)."""
+        self._check(lib().swe_hip_adj_set_launch(self._a, int(x_rows), int(y_rows)))
 
     def info(self):
-        n, f, b = C.c_int32(), C.c_double(), C.c_double()
-        self._check(lib().swe_hip_adj_info(self._a, C.byref(n), C.byref(f), C.byref(b)))
-        return {"nsnap": n.value, "forward_ms": f.value, "backward_ms": b.value}
+        n, r, f, b = C.c_int32(), C.c_int32(), C.c_double(), C.c_double()
+        self._check(lib().swe_hip_adj_info(self._a, C.byref(n), C.byref(r), C.byref(f),
+                                           C.byref(b)))
+        return {"nsnap": n.value, "records": bool(r.value), "forward_ms": f.value,
+                "backward_ms": b.value}

@@ -53,7 +53,7 @@ def gpu_gradient(hip, case, bc, nsteps, dt, lam, nsnap=2, tile_rows=None):
         s.upload(case["h"], case["hu"], case["hv"], case["B"])
         with hip.HipAdjoint(s, nsteps, nsnap) as adj:
             if tile_rows:
-                adj.set_tile_rows(tile_rows)
+                adj.set_launch(tile_rows, max(1, tile_rows // 2))
             adj.forward(dt=dt)
             final = s.download()
             grads = adj.backward(*lam)
@@ -149,7 +149,7 @@ def test_gradient_independent_of_checkpoints(hip):
     rng = np.random.default_rng(3)
     lam = [interior(case["h"].shape, rng) for _ in range(3)]
     ref = None
-    for nsnap in (9, 1, 2, 3, 5):
+    for nsnap in (9, 1, 2, 3, 5, -1):
         _, g = gpu_gradient(hip, case, 0, 9, dt, lam, nsnap=nsnap)
         if ref is None:
             ref = g
@@ -158,13 +158,13 @@ def test_gradient_independent_of_checkpoints(hip):
                 assert same(a, b), nsnap
 
 
-def test_gradient_independent_of_tile_rows(hip):
+def test_gradient_independent_of_launch_shape(hip):
     case = mild_state(50, 61, 12, bc=1)
     _, _, dt = setup(case, 1)
     rng = np.random.default_rng(4)
     lam = [interior(case["h"].shape, rng) for _ in range(3)]
     _, ref = gpu_gradient(hip, case, 1, 3, dt, lam, tile_rows=16)
-    for t in (1, 2, 7, 64):
+    for t in (1, 2, 7, 49, 64):
         _, g = gpu_gradient(hip, case, 1, 3, dt, lam, tile_rows=t)
         for a, b in zip(g, ref):
             assert same(a, b), t
