@@ -12,7 +12,11 @@ events recorded on the solver's stream around every stage launch in a second tim
 `cpu_baseline` times the reference's own OpenMP kernels (oracle/_ref) on the host on a
 few steps of the same grid; those steps are also compared bitwise with the GPU.
 
+Also reported: `batched` (512-instance ensemble vs per-instance launches) and `adjoint`
+(1024^2, 2000 steps: adjoint wall time over forward wall time).
+
     python bench.py [--steps K] [--warmup W] [--n 8192] [--cpu-steps S] [--no-cpu]
+                    [--no-ensemble] [--no-adjoint]
 """
 from __future__ import annotations
 
@@ -137,6 +141,53 @@ def ensemble_bench(steps, base_steps, check_every=32):
     }
 
 
+def adjoint_bench(n=1024, steps=2000):
+    """Config 5: 1024^2, 2000 SSP-RK3 steps (fixed dt from CFL 0.4 at t = 0), gauge-misfit
+    loss at 16 seeded cells.  Wall time of the plain forward (graph-replayed steps) vs the
+    adjoint (forward with checkpoints + backward), for the automatic storage choice and for a
+    binomial schedule with 64 stored states."""
+    import swe_hip
+    from swe_hip import workloads as W
+
+    case = W.adjoint_case(n, steps=steps)
+    rows, cols = W.adjoint_gauges(n)
+    with swe_hip.HipSolver.from_case(case) as s:
+        s.upload(case["h"], case["hu"], case["hv"], case["B"])
+        dt, _ = s.stable_dt(0.4)
+        s.step(dt, 20)
+        s.upload(case["h"], case["hu"], case["hv"], case["B"])
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(dt, steps)
+        t_fwd = time.perf_counter() - t0
+        target = s.download()[0][rows, cols] * 0.99  # a synthetic target: 1% lower gauges
+    res = {"workload": f"adjoint_{n}x{n}_{steps}_steps (16-bump bathymetry, surface bump, "
+                       "background undulation + flow), gauge-misfit loss at 16 cells",
+           "forward_s": round(t_fwd, 4)}
+    for label, nsnap in (("auto", 0), ("binomial_64", 64)):
+        with swe_hip.HipSolver.from_case(case) as s:
+            s.upload(case["h"], case["hu"], case["hv"], case["B"])
+            with swe_hip.HipAdjoint(s, steps, nsnap) as adj:
+                t0 = time.perf_counter()
+                adj.forward(dt=dt)
+                hN = s.download()[0]
+                lam = np.zeros_like(hN)
+                np.add.at(lam, (rows, cols), 2.0 * (hN[rows, cols] - target))
+                z = np.zeros_like(hN)
+                g = adj.backward(lam, z, z)
+                t_adj = time.perf_counter() - t0
+                info = adj.info()
+        res[label] = {"adjoint_s": round(t_adj, 4), "ratio": round(t_adj / t_fwd, 3),
+                      "nsnap": info["nsnap"], "stage_records": info["records"],
+                      "forward_ms": round(info["forward_ms"], 1),
+                      "backward_ms": round(info["backward_ms"], 1),
+                      "grad_finite": bool(np.isfinite(g[0]).all() and np.isfinite(g[3]).all())}
+        log(f"[bench] adjoint {label}: {t_adj:.3f} s vs forward {t_fwd:.3f} s "
+            f"-> {t_adj / t_fwd:.2f}x ({info})")
+    res["ratio"] = res["auto"]["ratio"]
+    return res
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -150,11 +201,13 @@ def main():
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--ensemble-base-steps", type=int, default=30)
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--adjoint-steps", type=int, default=2000)
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py measures one GPU only: the multi-GPU slab decomposition is a design "
-              "(DESIGN.md), not part of this build. Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU only: the slab decomposition across GPUs is not "
+              "implemented. Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import swe_hip
@@ -254,6 +307,8 @@ def main():
             log(f"[bench] CPU baseline skipped: {e}")
     if not args.no_ensemble:
         out["batched"] = ensemble_bench(args.ensemble_steps, args.ensemble_base_steps)
+    if not args.no_adjoint:
+        out["adjoint"] = adjoint_bench(steps=args.adjoint_steps)
     print(json.dumps(out), flush=True)
 
 

@@ -148,7 +148,18 @@ def lake_wet_dry(n: int = 4096, seed: int = 2, bump: bool = True, steps: int = 2
 
 
 def adjoint_case(n: int = 1024, seed: int = 5, steps: int = 2000, g: float = 9.80665,
-                 L: float = 10.0, nbumps: int = 16) -> Case:
+                 L: float = 10.0, nbumps: int = 16, background: bool = True) -> Case:
+    """Adjoint workload: 16 seeded Gaussian bumps of bathymetry (A in [0.05, 0.3], width in
+    [0.5, 1.5]), surface eta = 1 + 0.1 * (Gaussian bump of width 0.5 at a seeded place),
+    reflecting walls.
+
+    background=True adds a smooth surface undulation (0.01 sin*cos + a 0.003 ramp) and a
+    background flow (u = 0.1 sin + 0.02, v = 0.1 sin - 0.03).  Without it, most of the lake
+    is exactly at rest with eta = 1.0 to the last bit: neighbouring differences are exactly
+    0, where minmod's directional derivative minmod(da, db) is not linear in the direction,
+    so the discrete map has no derivative there (and the branch-taken linearization of the
+    rounding-level staircase at the bump's edge grows without bound over 2000 steps).
+    """
     c = _case(n, n, L, L, g, 0, 0.4, steps, name=f"adjoint_{n}")
     ng = c["ng"]
     x = centres(n, L)
@@ -160,11 +171,30 @@ def adjoint_case(n: int = 1024, seed: int = 5, steps: int = 2000, g: float = 9.8
     bx, by = uniform(seed, 2, 25, 0.3 * L, 0.7 * L)
     eta = 1.0 + 0.1 * np.outer(np.exp(-((x - bx) ** 2) / (2 * 0.5 ** 2)),
                                np.exp(-((x - by) ** 2) / (2 * 0.5 ** 2)))
+    if background:
+        eta = eta + 0.01 * np.outer(np.sin(2 * np.pi * x / L + 0.3),
+                                    np.cos(2 * np.pi * x / L + 0.7)) + 0.003 * (x[:, None] / L)
     c["B"][ng:ng + n, ng:ng + n] = Bi
-    c["h"][ng:ng + n, ng:ng + n] = np.maximum(0.0, eta - Bi)
+    h = np.maximum(0.0, eta - Bi)
+    c["h"][ng:ng + n, ng:ng + n] = h
+    if background:
+        u = 0.1 * np.outer(np.ones(n), np.sin(2 * np.pi * x / L + 1.1)) + 0.02
+        v = 0.1 * np.outer(np.sin(2 * np.pi * x / L + 0.4), np.ones(n)) - 0.03
+        c["hu"][ng:ng + n, ng:ng + n] = h * u
+        c["hv"][ng:ng + n, ng:ng + n] = h * v
     fill_ghosts(c["B"], ng, 0)
     fill_ghosts(c["h"], ng, 0)
+    fill_ghosts(c["hu"], ng, 0, sign_x=-1.0)
+    fill_ghosts(c["hv"], ng, 0, sign_y=-1.0)
     return c
+
+
+def adjoint_gauges(n: int, seed: int = 5, count: int = 16):
+    """Seeded gauge cells (row, column index arrays in the padded layout, ng = 2)."""
+    r = uniform(seed, 2 * count, 26, 0.0, 1.0)
+    rows = 2 + np.minimum((r[:count] * n).astype(np.int64), n - 1)
+    cols = 2 + np.minimum((r[count:] * n).astype(np.int64), n - 1)
+    return rows, cols
 
 
 def ensemble_sides(count: int = 512, seed: int = 4, lo: int = 64, hi: int = 1024) -> np.ndarray:

@@ -223,3 +223,92 @@ def test_torch_autograd_wrapper(hip):
     L.backward()
     for k, want in zip(("h", "hu", "hv", "B"), (gh, ghu, ghv, gb)):
         assert same(t[k].grad.cpu().numpy(), want), k
+
+
+@pytest.mark.slow
+def test_config5_finite_differences(hip):
+    """1024^2, 2000 steps, gauge-misfit loss: the adjoint's directional derivatives against
+    central differences of the GPU forward on 8 smooth random directions in (eta0, b).  The
+    discrete map is only piecewise smooth (minmod, wave-speed and branch kinks everywhere),
+    so agreement is limited by kink crossings; the bound here is what that allows."""
+    n, steps = 1024, 2000
+    case = W.adjoint_case(n, steps=steps)
+    rows, cols = W.adjoint_gauges(n)
+    with hip.HipSolver.from_case(case) as s:
+        s.upload(case["h"], case["hu"], case["hv"], case["B"])
+        dt, _ = s.stable_dt(0.4)
+
+    def run(h0, B):
+        with hip.HipSolver.from_case(case) as s:
+            s.upload(h0, case["hu"], case["hv"], B)
+            s.step(dt, steps)
+            return s.download()[0][rows, cols]
+
+    target = run(case["h"], case["B"]) * 0.99
+    with hip.HipSolver.from_case(case) as s:
+        s.upload(case["h"], case["hu"], case["hv"], case["B"])
+        with hip.HipAdjoint(s, steps, 0) as adj:
+            adj.forward(dt=dt)
+            hN = s.download()[0]
+            lam = np.zeros_like(hN)
+            np.add.at(lam, (rows, cols), 2.0 * (hN[rows, cols] - target))
+            z = np.zeros_like(hN)
+            gh, _, _, gb = adj.backward(lam, z, z)
+    g_eta, g_b = gh, gb - gh  # h0 = eta0 - b (wet everywhere)
+    assert (case["h"][2:-2, 2:-2] > 0.1).all()
+
+    L = 10.0
+    x = (np.arange(n) + 0.5) * L / n
+    errs = []
+    for k in range(8):
+        r = np.random.default_rng(100 + k)
+        dirs = []
+        for _ in range(2):
+            f = np.zeros((n, n))
+            for _ in range(4):
+                kx, ky = r.integers(1, 4, 2)
+                px, py = r.uniform(0, 2 * np.pi, 2)
+                f += r.standard_normal() * np.outer(np.sin(2 * np.pi * kx * x / L + px),
+                                                    np.sin(2 * np.pi * ky * x / L + py))
+            a = np.zeros_like(case["h"])
+            a[2:-2, 2:-2] = f / np.abs(f).max()
+            dirs.append(a)
+        de, db = dirs[0], 0.3 * dirs[1]
+        gv = float((g_eta * de).sum() + (g_b * db).sum())
+        eps = 1e-8
+        lp = float(np.sum((run(case["h"] + eps * (de - db), case["B"] + eps * db) - target) ** 2))
+        lm = float(np.sum((run(case["h"] - eps * (de - db), case["B"] - eps * db) - target) ** 2))
+        fd = (lp - lm) / (2 * eps)
+        errs.append(abs(fd - gv) / abs(gv))
+    print("relative |FD - adjoint| on 8 directions:", errs)
+    assert np.median(errs) < 1e-4 and max(errs) < 1e-3, errs
+
+
+@pytest.mark.slow
+def test_config5_matches_tangent_at_128(hip):
+    """The config-5 workload at 128^2 over 2000 steps: <lam, J v> from the CPU dual-number
+    tangent equals <J^T lam, v> from the GPU adjoint to rounding."""
+    n, steps = 128, 2000
+    case = W.adjoint_case(n, steps=steps)
+    rows, cols = W.adjoint_gauges(n)
+    gr, B, _ = setup(case, 0)
+    with hip.HipSolver.from_case(case) as s:
+        s.upload(case["h"], case["hu"], case["hv"], case["B"])
+        dt, _ = s.stable_dt(0.4)
+        with hip.HipAdjoint(s, steps, 16) as adj:   # a binomial schedule
+            adj.forward(dt=dt)
+            hN = s.download()[0]
+            lam = np.zeros_like(hN)
+            lam[rows, cols] = 1.0
+            z = np.zeros_like(hN)
+            gh, ghu, ghv, gb = adj.backward(lam, z, z)
+    rng = np.random.default_rng(9)
+    shape = case["h"].shape
+    for k in range(2):
+        v = [interior(shape, rng) for _ in range(4)]
+        x_ = [case[q].copy() for q in ("h", "hu", "hv")]
+        d = [a.copy() for a in v[:3]]
+        assert oracle.tangent_run(gr, *x_, B, *d, v[3], np.full(steps, dt)) == 0
+        lhs = float(d[0][rows, cols].sum())
+        rhs = float((gh * v[0]).sum() + (ghu * v[1]).sum() + (ghv * v[2]).sum() + (gb * v[3]).sum())
+        assert abs(lhs - rhs) <= 1e-9 * max(abs(lhs), 1e-300), (lhs, rhs)

@@ -98,7 +98,7 @@ def test_tangent_is_linear_in_the_direction():
 def test_flat_lake_is_not_differentiable():
     """Exact ties make central differences disagree with any linear derivative: a lake at
     rest (eta = 1.0 exactly away from the bump) perturbed along a smooth direction."""
-    case = W.adjoint_case(24, steps=10)
+    case = W.adjoint_case(24, steps=10, background=False)
     gr, B, dt = prepared(case, 0, 0.4)
     dts = np.full(10, dt)
     shape = case["h"].shape
