@@ -38,6 +38,11 @@
 // their adjoints are folded back into the cells they mirror, with the reflecting walls'
 // sign flips (hu on x ghosts, hv on y ghosts).
 //
+// Launches: adjoint_stage_kernel does a whole stage in one launch, the x faces of a block
+// of rows passing through LDS (the default); adjoint_xface_kernel + adjoint_y_kernel do it
+// in two, the x faces passing through device memory.  Both sum every cell's terms in the
+// same order, so they give the same bits.
+//
 // HBM traffic per cell and stage: read V (24 B), B (8), lam (24), the output's floor mask
 // (1), write dL/dV (24), the floor mask of V (1), U^n accumulator (24 written; stage 2 also
 // reads it, stage 1 only reads it), dL/dB read + write (16): 122 / 146 / 122 bytes for
@@ -438,64 +443,124 @@ __device__ __forceinline__ double stage_c() {
 // ---- x gather ------------------------------------------------------------------------
 // Sums the x faces' adjoint parts of a row r (rows ng-2 .. ng+nx+1; faces outside the
 // domain contribute nothing): the direct parts of faces r (minus side) and r-1 (plus side)
-// and the slopes of rows r-1, r, r+1 routed by their minmod selections.
-template <int BC>
-struct XGather {
-    const GridDev &G;
-    const AdjArgs &A;
-    int j;
-    long long plane;
-
-    __device__ __forceinline__ double xf(int k, int f) const {
-        return A.xf[k * plane + (long long)(f - (G.ng - 1)) * G.stride + j];
+// and the slopes of rows r-1, r, r+1 routed by their minmod selections.  S supplies the
+// face outputs xf(k, f) (k = em, nm, tm, bm, ep, np, tp, bp) and the selections sel(t);
+// the summation order is fixed here, so every source gives the same bits.
+template <class S>
+__device__ __forceinline__ A4 gather_row(const S &X, const GridDev &G, int r) {
+    auto face_ok = [&](int f) { return f >= G.ng - 1 && f <= G.ng + G.nx - 1; };
+    A4 T{0.0, 0.0, 0.0, 0.0};
+    if (face_ok(r)) {
+        T.e += X.xf(0, r);
+        T.u += X.xf(1, r);
+        T.v += X.xf(2, r);
+        T.b += X.xf(3, r);
     }
-    __device__ __forceinline__ bool face_ok(int f) const {
-        return f >= G.ng - 1 && f <= G.ng + G.nx - 1;
+    if (face_ok(r - 1)) {
+        T.e += X.xf(4, r - 1);
+        T.u += X.xf(5, r - 1);
+        T.v += X.xf(6, r - 1);
+        T.b += X.xf(7, r - 1);
+    }
+    for (int t = r - 1; t <= r + 1; ++t) {
+        if (t < G.ng - 1 || t > G.ng + G.nx) continue;
+        // slope adjoint of row t: +minus state of face t, -plus state of face t-1
+        double ge = 0.0, gu = 0.0, gv = 0.0;
+        if (face_ok(t)) {
+            ge += X.xf(0, t);
+            gu += X.xf(1, t);
+            gv += X.xf(2, t);
+        }
+        if (face_ok(t - 1)) {
+            ge -= X.xf(4, t - 1);
+            gu -= X.xf(5, t - 1);
+            gv -= X.xf(6, t - 1);
+        }
+        const int sl = X.sel(t);
+        double tp, ts, tn;
+        slope_parts(sl & 3, ge, tp, ts, tn);
+        T.e += t == r - 1 ? tn : (t == r ? ts : tp);
+        slope_parts((sl >> 2) & 3, gu, tp, ts, tn);
+        T.u += t == r - 1 ? tn : (t == r ? ts : tp);
+        slope_parts((sl >> 4) & 3, gv, tp, ts, tn);
+        T.v += t == r - 1 ? tn : (t == r ? ts : tp);
+    }
+    return T;
+}
+
+// x-face outputs in device memory (written by adjoint_xface_kernel).
+struct XGather {
+    const AdjArgs &A;
+    int ng, stride, j;
+    long long plane;
+    __device__ __forceinline__ double xf(int k, int f) const {
+        return A.xf[k * plane + (long long)(f - (ng - 1)) * stride + j];
     }
     __device__ __forceinline__ int sel(int t) const {
-        return A.xsel[(long long)(t - (G.ng - 1)) * G.stride + j];
-    }
-
-    __device__ __forceinline__ A4 row(int r) const {
-        A4 T{0.0, 0.0, 0.0, 0.0};
-        if (face_ok(r)) {
-            T.e += xf(0, r);
-            T.u += xf(1, r);
-            T.v += xf(2, r);
-            T.b += xf(3, r);
-        }
-        if (face_ok(r - 1)) {
-            T.e += xf(4, r - 1);
-            T.u += xf(5, r - 1);
-            T.v += xf(6, r - 1);
-            T.b += xf(7, r - 1);
-        }
-        for (int t = r - 1; t <= r + 1; ++t) {
-            if (t < G.ng - 1 || t > G.ng + G.nx) continue;
-            // slope adjoint of row t: +minus state of face t, -plus state of face t-1
-            double ge = 0.0, gu = 0.0, gv = 0.0;
-            if (face_ok(t)) {
-                ge += xf(0, t);
-                gu += xf(1, t);
-                gv += xf(2, t);
-            }
-            if (face_ok(t - 1)) {
-                ge -= xf(4, t - 1);
-                gu -= xf(5, t - 1);
-                gv -= xf(6, t - 1);
-            }
-            const int sl = sel(t);
-            double tp, ts, tn;
-            slope_parts(sl & 3, ge, tp, ts, tn);
-            T.e += t == r - 1 ? tn : (t == r ? ts : tp);
-            slope_parts((sl >> 2) & 3, gu, tp, ts, tn);
-            T.u += t == r - 1 ? tn : (t == r ? ts : tp);
-            slope_parts((sl >> 4) & 3, gv, tp, ts, tn);
-            T.v += t == r - 1 ? tn : (t == r ? ts : tp);
-        }
-        return T;
+        return A.xsel[(long long)(t - (ng - 1)) * stride + j];
     }
 };
+
+// x-face outputs of a block's faces F0 .. F1 in LDS (adjoint_stage_kernel): face f at
+// [(f - F0) * 8 + k][lane], the selections of rows F0 .. F1+1 after them.
+struct LdsGather {
+    const double *faces;
+    const unsigned char *sels;
+    int F0, lane;
+    __device__ __forceinline__ double xf(int k, int f) const {
+        return faces[((f - F0) * 8 + k) * kWave + lane];
+    }
+    __device__ __forceinline__ int sel(int t) const { return sels[(t - F0) * kWave + lane]; }
+};
+
+// The ghost rows' x parts folded into the interior rows they mirror (reflecting: ng-1 ->
+// ng, ng-2 -> ng+1; transmissive: both -> ng; likewise at the far wall).
+template <int BC, class S>
+__device__ __forceinline__ A4 gather_folded(const S &X, const GridDev &G, int r) {
+    const int ng = G.ng, nx = G.nx;
+    A4 T = gather_row(X, G, r);
+    if (r == ng) {
+        fold_row<BC>(T, gather_row(X, G, ng - 1));
+        if (BC != 0) fold_row<BC>(T, gather_row(X, G, ng - 2));
+    }
+    if (BC == 0 && r == ng + 1) fold_row<BC>(T, gather_row(X, G, ng - 2));
+    if (r == ng + nx - 1) {
+        fold_row<BC>(T, gather_row(X, G, ng + nx));
+        if (BC != 0) fold_row<BC>(T, gather_row(X, G, ng + nx + 1));
+    }
+    if (BC == 0 && r == ng + nx - 2) fold_row<BC>(T, gather_row(X, G, ng + nx + 1));
+    return T;
+}
+
+// The stage's outputs at an owned cell of row r: dL/dV (stage 1: plus the U^n accumulator),
+// the U^n accumulator, dL/dB.  acc = the y faces' part, T = the x faces' part.
+template <int STAGE, int BC>
+__device__ __forceinline__ void emit_cell(const AdjArgs &A, const Lanes<BC> &w, int r,
+                                          const D3 &L, const A4 &acc, const A4 &T) {
+    constexpr double cv = STAGE == 1 ? 1.0 : (STAGE == 2 ? 0.25 : 2.0 / 3.0);
+    const long long ro = w.row_off(r);
+    const unsigned jb = w.j_bytes;
+    double oh = cv * L.e + (T.e + acc.e);
+    double ou = cv * L.u + (T.u + acc.u);
+    double ov = cv * L.v + (T.v + acc.v);
+    if constexpr (STAGE == 3) {
+        st(row_ptr(A.e_h, ro), jb, (1.0 / 3.0) * L.e);
+        st(row_ptr(A.e_hu, ro), jb, (1.0 / 3.0) * L.u);
+        st(row_ptr(A.e_hv, ro), jb, (1.0 / 3.0) * L.v);
+    } else if constexpr (STAGE == 2) {
+        st(row_ptr(A.e_h, ro), jb, ld(row_ptr(A.e_h, ro), jb) + 0.75 * L.e);
+        st(row_ptr(A.e_hu, ro), jb, ld(row_ptr(A.e_hu, ro), jb) + 0.75 * L.u);
+        st(row_ptr(A.e_hv, ro), jb, ld(row_ptr(A.e_hv, ro), jb) + 0.75 * L.v);
+    } else {
+        oh += ld(row_ptr(A.e_h, ro), jb);
+        ou += ld(row_ptr(A.e_hu, ro), jb);
+        ov += ld(row_ptr(A.e_hv, ro), jb);
+    }
+    st(row_ptr(A.out_h, ro), jb, oh);
+    st(row_ptr(A.out_hu, ro), jb, ou);
+    st(row_ptr(A.out_hv, ro), jb, ov);
+    st(row_ptr(A.lam_b, ro), jb, ld(row_ptr(A.lam_b, ro), jb) + ((T.e + T.b) + (acc.e + acc.b)));
+}
 
 // ---- y faces, x gather and the stage's outputs: every row independently -----------------
 // Runs after adjoint_xface_kernel.  Per row: the y faces' adjoint (lanes, DPP), the x
@@ -515,7 +580,7 @@ void adjoint_y_kernel(const GridDev G, const AdjArgs A) {
     const int r1 = (r0 + A.y_rows < ng + nx) ? r0 + A.y_rows : ng + nx;
     constexpr double cv = STAGE == 1 ? 1.0 : (STAGE == 2 ? 0.25 : 2.0 / 3.0);
     const double ky = G.inv_dy * (cv * *A.dt);
-    const XGather<BC> X{G, A, (int)(w.j_bytes >> 3), (long long)(nx + 1) * G.stride};
+    const XGather X{A, ng, G.stride, (int)(w.j_bytes >> 3), (long long)(nx + 1) * G.stride};
     bool fl;
     PCell P = w.load(r0, fl);
     D3 L = w.load_lam(r0);
@@ -527,42 +592,7 @@ void adjoint_y_kernel(const GridDev G, const AdjArgs A) {
         A4 acc = (G.split_ok && __all(Lanes<BC>::split_row(P)))
                      ? w.template y_row<true>(G.half_g, G.g, ky, P, L)
                      : w.template y_row<false>(G.half_g, G.g, ky, P, L);
-        if (w.owner) {
-            A4 T = X.row(r);
-            if (r == ng) {
-                fold_row<BC>(T, X.row(ng - 1));
-                if (BC != 0) fold_row<BC>(T, X.row(ng - 2));
-            }
-            if (BC == 0 && r == ng + 1) fold_row<BC>(T, X.row(ng - 2));
-            if (r == ng + nx - 1) {
-                fold_row<BC>(T, X.row(ng + nx));
-                if (BC != 0) fold_row<BC>(T, X.row(ng + nx + 1));
-            }
-            if (BC == 0 && r == ng + nx - 2) fold_row<BC>(T, X.row(ng + nx + 1));
-            const long long ro = w.row_off(r);
-            const unsigned jb = w.j_bytes;
-            double oh = cv * L.e + (T.e + acc.e);
-            double ou = cv * L.u + (T.u + acc.u);
-            double ov = cv * L.v + (T.v + acc.v);
-            if constexpr (STAGE == 3) {
-                st(row_ptr(A.e_h, ro), jb, (1.0 / 3.0) * L.e);
-                st(row_ptr(A.e_hu, ro), jb, (1.0 / 3.0) * L.u);
-                st(row_ptr(A.e_hv, ro), jb, (1.0 / 3.0) * L.v);
-            } else if constexpr (STAGE == 2) {
-                st(row_ptr(A.e_h, ro), jb, ld(row_ptr(A.e_h, ro), jb) + 0.75 * L.e);
-                st(row_ptr(A.e_hu, ro), jb, ld(row_ptr(A.e_hu, ro), jb) + 0.75 * L.u);
-                st(row_ptr(A.e_hv, ro), jb, ld(row_ptr(A.e_hv, ro), jb) + 0.75 * L.v);
-            } else {
-                oh += ld(row_ptr(A.e_h, ro), jb);
-                ou += ld(row_ptr(A.e_hu, ro), jb);
-                ov += ld(row_ptr(A.e_hv, ro), jb);
-            }
-            st(row_ptr(A.out_h, ro), jb, oh);
-            st(row_ptr(A.out_hu, ro), jb, ou);
-            st(row_ptr(A.out_hv, ro), jb, ov);
-            st(row_ptr(A.lam_b, ro), jb,
-               ld(row_ptr(A.lam_b, ro), jb) + ((T.e + T.b) + (acc.e + acc.b)));
-        }
+        if (w.owner) emit_cell<STAGE, BC>(A, w, r, L, acc, gather_folded<BC>(X, G, r));
         P = Pn;
         L = Ln;
     }
@@ -669,6 +699,130 @@ void adjoint_xface_kernel(const GridDev G, const AdjArgs A) {
     }
 }
 
+// ---- the whole adjoint stage in one launch --------------------------------------------
+// A block of W waves covers one strip and the rows R0 .. R0 + W*T - 1 (T = tile_rows).
+// Phase 1: the block's x faces F0 = max(R0-2, ng-1) .. F1 = min(R0 + W*T, ng+nx-1) -- the
+// faces its rows' x parts depend on -- are split into W contiguous runs, one per wave; each
+// wave marches its run as adjoint_xface_kernel does and writes every face's 8 outputs and
+// each row's minmod selections to LDS.  Phase 2, after the barrier: each wave takes T rows,
+// reverses their y faces (lanes, DPP), gathers the x parts from LDS with the ghost-row
+// folds, and writes the outputs and the floor mask of V.  Faces within 2 rows of a block
+// edge are computed by both neighbouring blocks (3 faces per block).
+// LDS: (W*T + 3) faces x 8 x 64 doubles, then (W*T + 4) x 64 selection bytes.
+template <int STAGE, int BC, int W>
+__global__ __launch_bounds__(kWave * W) __attribute__((amdgpu_waves_per_eu(SWE_ADJ_WAVES_PER_EU)))
+void adjoint_stage_kernel(const GridDev G, const AdjArgs A) {
+    extern __shared__ double lds_faces[];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
+    const Lanes<BC> w(G, A, (int)blockIdx.x, lane);
+    const int ng = G.ng, nx = G.nx, T = A.tile_rows;
+    const int R0 = ng + (int)blockIdx.y * (W * T);
+    const int Rend = (R0 + W * T < ng + nx) ? R0 + W * T : ng + nx;
+    const int F0 = (R0 - 2 > ng - 1) ? R0 - 2 : ng - 1;
+    const int F1 = (Rend < ng + nx - 1) ? Rend : ng + nx - 1;
+    const int NF = F1 - F0 + 1;
+    unsigned char *lds_sel =
+        reinterpret_cast<unsigned char *>(lds_faces + (long)(W * T + 3) * 8 * kWave);
+    constexpr double cv = STAGE == 1 ? 1.0 : (STAGE == 2 ? 0.25 : 2.0 / 3.0);
+
+    // ---- phase 1: x faces fa .. fb-1 of this wave
+    {
+        const int fa = F0 + (wv * NF) / W, fb = F0 + ((wv + 1) * NF) / W;
+        if (fa < fb) {
+            const double kx = G.inv_dx * (cv * *A.dt);
+            bool split = G.split_ok != 0;
+            bool fl;
+            PCell P0, P1, P2, P3;
+            D3 S0, S1;
+            int sel0, sel1;
+            {
+                const PCell Pm = w.load(fa - 1, fl);
+                P0 = w.load(fa, fl);
+                P1 = w.load(fa + 1, fl);
+                P2 = w.load(fa + 2, fl);
+                split = split && __all(Lanes<BC>::split_row(Pm) && Lanes<BC>::split_row(P0) &&
+                                       Lanes<BC>::split_row(P1) && Lanes<BC>::split_row(P2));
+                int sa, sb, sc;
+                S0.e = half_minmod(P0.eta - Pm.eta, P1.eta - P0.eta, sa);
+                S0.u = half_minmod(P0.hu - Pm.hu, P1.hu - P0.hu, sb);
+                S0.v = half_minmod(P0.hv - Pm.hv, P1.hv - P0.hv, sc);
+                sel0 = sa | (sb << 2) | (sc << 4);
+            }
+            D3 L0 = w.load_lam(fa), L1 = w.load_lam(fa + 1), L2;
+#pragma unroll 1
+            for (int f = fa; f < fb; ++f) {
+                const int rn = f + 3 <= ng + nx + 1 ? f + 3 : ng + nx + 1;
+                P3 = w.load(rn, fl);
+                L2 = w.load_lam(f + 2);
+                int sa, sb, sc;
+                S1.e = half_minmod(P1.eta - P0.eta, P2.eta - P1.eta, sa);
+                S1.u = half_minmod(P1.hu - P0.hu, P2.hu - P1.hu, sb);
+                S1.v = half_minmod(P1.hv - P0.hv, P2.hv - P1.hv, sc);
+                sel1 = sa | (sb << 2) | (sc << 4);
+                // x faces: normal momentum hu, tangential hv
+                FaceAdj x;
+                if (split)
+                    x = face_adjoint<true>(G.half_g, G.g, P0.eta + S0.e, P0.hu + S0.u,
+                                           P0.hv + S0.v, P0.b, P1.eta - S1.e, P1.hu - S1.u,
+                                           P1.hv - S1.v, P1.b, -kx * L0.e, -kx * L0.u,
+                                           -kx * L0.v, kx * L1.e, kx * L1.u, kx * L1.v);
+                else
+                    x = face_adjoint<false>(G.half_g, G.g, P0.eta + S0.e, P0.hu + S0.u,
+                                            P0.hv + S0.v, P0.b, P1.eta - S1.e, P1.hu - S1.u,
+                                            P1.hv - S1.v, P1.b, -kx * L0.e, -kx * L0.u,
+                                            -kx * L0.v, kx * L1.e, kx * L1.u, kx * L1.v);
+                double *slot = lds_faces + (f - F0) * 8 * kWave + lane;
+                slot[0 * kWave] = x.em;
+                slot[1 * kWave] = x.nm;
+                slot[2 * kWave] = x.tm;
+                slot[3 * kWave] = x.bm;
+                slot[4 * kWave] = x.ep;
+                slot[5 * kWave] = x.np;
+                slot[6 * kWave] = x.tp;
+                slot[7 * kWave] = x.bp;
+                lds_sel[(f - F0) * kWave + lane] = (unsigned char)sel0;
+                if (f == F1) lds_sel[(f + 1 - F0) * kWave + lane] = (unsigned char)sel1;
+                split = split && __all(Lanes<BC>::split_row(P3));
+                P0 = P1;
+                P1 = P2;
+                P2 = P3;
+                S0 = S1;
+                sel0 = sel1;
+                L0 = L1;
+                L1 = L2;
+            }
+        }
+    }
+    __syncthreads();
+
+    // ---- phase 2: rows ra .. rb-1 of this wave
+    const int ra = R0 + wv * T;
+    const int rb = (ra + T < Rend) ? ra + T : Rend;
+    if (ra >= rb) return;
+    const double ky = G.inv_dy * (cv * *A.dt);
+    const LdsGather X{lds_faces, lds_sel, F0, lane};
+    bool fl, fln;
+    PCell P = w.load(ra, fl);
+    D3 L = w.load_lam(ra);
+#pragma unroll 1
+    for (int r = ra; r < rb; ++r) {
+        const int rn = r + 1 < rb ? r + 1 : r;
+        const PCell Pn = w.load(rn, fln);
+        const D3 Ln = w.load_lam(rn);
+        A4 acc = (G.split_ok && __all(Lanes<BC>::split_row(P)))
+                     ? w.template y_row<true>(G.half_g, G.g, ky, P, L)
+                     : w.template y_row<false>(G.half_g, G.g, ky, P, L);
+        if (w.owner) {
+            A.mask_in[w.row_off(r) + (w.j_bytes >> 3)] = fl ? 1 : 0;
+            emit_cell<STAGE, BC>(A, w, r, L, acc, gather_folded<BC>(X, G, r));
+        }
+        P = Pn;
+        L = Ln;
+        fl = fln;
+    }
+}
+
 // Floor-state flags of a state's interior cells (the adjoint of the last step's stage 3).
 __global__ __launch_bounds__(256) void floor_mask_kernel(const GridDev G, const double *h,
                                                          const double *hu, const double *hv,
@@ -718,12 +872,36 @@ hipError_t launch_adj_one(const GridDev &G, const AdjArgs &A, hipStream_t stream
     return hipGetLastError();
 }
 
+template <int STAGE, int BC, int W>
+hipError_t launch_fused_w(const GridDev &G, const AdjArgs &A, hipStream_t stream) {
+    const size_t lds = adj_lds_bytes(W, A.tile_rows);
+    const void *fn = reinterpret_cast<const void *>(&adjoint_stage_kernel<STAGE, BC, W>);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL((adjoint_stage_kernel<STAGE, BC, W>),
+                       dim3((unsigned)A.nstrips,
+                            (unsigned)((G.nx + W * A.tile_rows - 1) / (W * A.tile_rows))),
+                       dim3(kWave * W), lds, stream, G, A);
+    return hipGetLastError();
+}
+
+template <int STAGE, int BC>
+hipError_t launch_stage(const GridDev &G, const AdjArgs &A, hipStream_t stream) {
+    switch (A.block_waves) {
+        case 0: return launch_adj_one<STAGE, BC>(G, A, stream);
+        case 2: return launch_fused_w<STAGE, BC, 2>(G, A, stream);
+        case 4: return launch_fused_w<STAGE, BC, 4>(G, A, stream);
+        case 8: return launch_fused_w<STAGE, BC, 8>(G, A, stream);
+        default: return hipErrorInvalidValue;
+    }
+}
+
 template <int BC>
 hipError_t launch_adj_bc(int stage, const GridDev &G, const AdjArgs &A, hipStream_t stream) {
     switch (stage) {
-        case 1: return launch_adj_one<1, BC>(G, A, stream);
-        case 2: return launch_adj_one<2, BC>(G, A, stream);
-        case 3: return launch_adj_one<3, BC>(G, A, stream);
+        case 1: return launch_stage<1, BC>(G, A, stream);
+        case 2: return launch_stage<2, BC>(G, A, stream);
+        case 3: return launch_stage<3, BC>(G, A, stream);
         default: return hipErrorInvalidValue;
     }
 }
@@ -732,11 +910,24 @@ hipError_t launch_adj_bc(int stage, const GridDev &G, const AdjArgs &A, hipStrea
 
 int adj_strips_for(int ny) { return (ny + kAdjCols - 1) / kAdjCols; }
 
+size_t adj_lds_bytes(int block_waves, int tile_rows) {
+    const size_t faces = (size_t)block_waves * tile_rows + 3;
+    return faces * 8 * kWave * sizeof(double) + (faces + 1) * kWave;
+}
+
+bool adj_launch_ok(int block_waves, int tile_rows) {
+    if (tile_rows < 1) return false;
+    if (block_waves == 0) return true;
+    if (block_waves != 2 && block_waves != 4 && block_waves != 8) return false;
+    return adj_lds_bytes(block_waves, tile_rows) <= kAdjMaxLds;
+}
+
 hipError_t launch_adjoint_stage(int stage, int bc, const GridDev &G, const AdjArgs &A,
                                 hipStream_t stream) {
-    if (G.ng != 2 || A.tile_rows < 1 || A.y_rows < 1 || A.nstrips != adj_strips_for(G.ny) ||
-        A.xf == nullptr || A.xsel == nullptr)
+    if (G.ng != 2 || !adj_launch_ok(A.block_waves, A.tile_rows) || A.y_rows < 1 ||
+        A.nstrips != adj_strips_for(G.ny))
         return hipErrorInvalidValue;
+    if (A.block_waves == 0 && (A.xf == nullptr || A.xsel == nullptr)) return hipErrorInvalidValue;
     return bc == 0 ? launch_adj_bc<0>(stage, G, A, stream) : launch_adj_bc<1>(stage, G, A, stream);
 }
 

@@ -106,13 +106,15 @@ struct swe_hip_adj {
     unsigned char *mask[2] = {nullptr, nullptr};
     double *dts = nullptr;
     double *partial = nullptr;  // 1024 per-block maxima (adaptive dt)
-    double *xf = nullptr;       // adjoint outputs of the x faces (8 per face and column)
-    unsigned char *xsel = nullptr;  // minmod selections of the x slopes
+    double *xf = nullptr;       // two-kernel path: x-face outputs (8 per face and column)
+    unsigned char *xsel = nullptr;  // two-kernel path: minmod selections of the x slopes
     long long *counter = nullptr;
     std::vector<Action> acts;
     size_t first_adj = 0;
-    int tile_rows = 4;   // x-face kernel faces per wave
-    int y_rows = 4;      // y-face kernel rows per wave (no halo cost: short is fine)
+    int block_waves = 4;  // fused kernel: waves per block (0 selects the two-kernel path)
+    int fused_rows = 3;   // fused kernel: rows per wave
+    int tile_rows = 4;    // two-kernel path: x faces per wave
+    int y_rows = 4;       // two-kernel path: rows per wave of the y-face kernel
     int state = 0;  // 0 created, 1 forward done (checkpoints valid), 2 consumed
     int parity = 0;
     double fwd_ms = 0.0, bwd_ms = 0.0;
@@ -240,7 +242,8 @@ swe::AdjArgs adj_args(swe_hip_adj *a, const State &v, int lam_in, int lam_out,
     r.lam_b = a->lam_b;
     r.mask_in = mask_in;
     r.dt = a->dts + p;
-    r.tile_rows = a->tile_rows;
+    r.block_waves = a->block_waves;
+    r.tile_rows = a->block_waves ? a->fused_rows : a->tile_rows;
     r.y_rows = a->y_rows;
     r.nstrips = swe::adj_strips_for(c->G.ny);
     r.xf = a->xf;
@@ -358,10 +361,6 @@ int swe_hip_adj_create(swe_hip_ctx *ctx, int64_t nsteps, int32_t nsnap, swe_hip_
     if (hipMalloc(&a->lam_b, bytes) != hipSuccess) return bad(SWE_ERR_OOM);
     for (auto &m : a->mask)
         if (hipMalloc(&m, a->n_total) != hipSuccess) return bad(SWE_ERR_OOM);
-    const size_t xfaces = (size_t)(ctx->G.nx + 1) * ctx->G.stride;
-    if (hipMalloc(&a->xf, 8 * xfaces * sizeof(double)) != hipSuccess ||
-        hipMalloc(&a->xsel, xfaces + ctx->G.stride) != hipSuccess)
-        return bad(SWE_ERR_OOM);
     if (hipMalloc(&a->dts, sizeof(double) * (size_t)nsteps) != hipSuccess ||
         hipMalloc(&a->partial, sizeof(double) * 1024) != hipSuccess ||
         hipMalloc(&a->counter, sizeof(long long)) != hipSuccess)
@@ -482,10 +481,32 @@ int swe_hip_adj_backward_device(swe_hip_adj *a, const double *lh, const double *
                            hipMemcpyDeviceToDevice);
 }
 
-int swe_hip_adj_set_launch(swe_hip_adj *a, int x_rows, int y_rows) {
-    if (!a || x_rows < 0 || y_rows < 0) return SWE_ERR_INVALID;
-    if (x_rows) a->tile_rows = x_rows;
-    if (y_rows) a->y_rows = y_rows;
+int swe_hip_adj_set_launch(swe_hip_adj *a, int block_waves, int rows, int y_rows) {
+    if (!a || rows < 0 || y_rows < 0) return SWE_ERR_INVALID;
+    if (block_waves == 0) {
+        if (!a->xf) {  // the x-face outputs of the two-kernel path, allocated on first use
+            swe_hip_ctx *c = a->ctx;
+            if (hipSetDevice(c->device) != hipSuccess) return afail(a, SWE_ERR_HIP, "hipSetDevice");
+            const size_t xfaces = (size_t)(c->G.nx + 1) * c->G.stride;
+            if (hipMalloc(&a->xf, 8 * xfaces * sizeof(double)) != hipSuccess ||
+                hipMalloc(&a->xsel, xfaces + c->G.stride) != hipSuccess) {
+                (void)hipFree(a->xf);
+                a->xf = nullptr;
+                a->xsel = nullptr;
+                return afail(a, SWE_ERR_OOM, "x-face buffers of the two-kernel path");
+            }
+        }
+        a->block_waves = 0;
+        if (rows) a->tile_rows = rows;
+        if (y_rows) a->y_rows = y_rows;
+        return SWE_OK;
+    }
+    const int r = rows ? rows : a->fused_rows;
+    if (!swe::adj_launch_ok(block_waves, r))
+        return afail(a, SWE_ERR_INVALID,
+                     "block_waves must be 2, 4 or 8 and its LDS tile within 160 KiB");
+    a->block_waves = block_waves;
+    a->fused_rows = r;
     return SWE_OK;
 }
 

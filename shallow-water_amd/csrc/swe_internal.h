@@ -122,14 +122,19 @@ struct AdjArgs {
     double *lam_b;
     unsigned char *mask_in;
     const double *dt;  // device scalar: this step's dt (frozen)
-    int tile_rows;     // x faces per wave of the x-face kernel
-    int y_rows;        // rows per wave of the y-face kernel
+    int tile_rows;     // rows per wave (fused kernel) / x faces per wave (x-face kernel)
+    int y_rows;        // rows per wave of the y-face kernel (two-kernel path)
+    int block_waves;   // waves per block of the fused kernel (4, 8, 16); 0: two kernels
     int nstrips;       // 58-column strips across y (adj_strips_for)
-    double *xf;        // 8 x (nx+1) x stride: adjoint outputs of every x face
-    unsigned char *xsel;  // (nx+2) x stride: minmod selections of rows ng-1 .. ng+nx
+    double *xf;        // two-kernel path: 8 x (nx+1) x stride adjoint outputs of the x faces
+    unsigned char *xsel;  // two-kernel path: (nx+2) x stride minmod selections
 };
 
 int adj_strips_for(int ny);
+// LDS of one fused-kernel block; launches above kAdjMaxLds are rejected.
+constexpr size_t kAdjMaxLds = 160 * 1024;
+size_t adj_lds_bytes(int block_waves, int tile_rows);
+bool adj_launch_ok(int block_waves, int tile_rows);
 // ng must be 2.
 hipError_t launch_adjoint_stage(int stage, int bc, const GridDev &G, const AdjArgs &A,
                                 hipStream_t stream);

@@ -102,7 +102,7 @@ _SIGS = {
     "swe_hip_adj_forward": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, _dp]),
     "swe_hip_adj_backward": (C.c_int, [C.c_void_p, _dp, _dp, _dp, _dp, _dp, _dp, _dp]),
     "swe_hip_adj_backward_device": (C.c_int, [C.c_void_p] + [C.c_void_p] * 7),
-    "swe_hip_adj_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "swe_hip_adj_set_launch": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
     "swe_hip_adj_schedule": (C.c_int, [C.c_int64, C.c_int32, C.POINTER(C.c_int64),
                                        C.POINTER(C.c_int32), C.POINTER(C.c_int64)]),
     "swe_hip_adj_info": (C.c_int, [C.c_void_p, C.POINTER(C.c_int32), C.POINTER(C.c_int32),
@@ -446,9 +446,12 @@ class HipAdjoint:
             self._a, lam_h.data_ptr(), lam_hu.data_ptr(), lam_hv.data_ptr(), g_h.data_ptr(),
             g_hu.data_ptr(), g_hv.data_ptr(), g_b.data_ptr() if g_b is not None else None))
 
-    def set_launch(self, x_rows: int = 0, y_rows: int = 0):
-        """Rows per wave of the x-march and y-face adjoint kernels (0 keeps the value)."""
-        self._check(lib().swe_hip_adj_set_launch(self._a, int(x_rows), int(y_rows)))
+    def set_launch(self, block_waves: int = 4, rows: int = 0, y_rows: int = 0):
+        """Launch shape of the adjoint stages: block_waves 2, 4 or 8 with `rows` rows per wave
+        (fused kernel), or 0 for the two-kernel path (`rows` x faces per wave, `y_rows`
+        rows per wave of the y kernel).  0 keeps a value."""
+        self._check(lib().swe_hip_adj_set_launch(self._a, int(block_waves), int(rows),
+                                                 int(y_rows)))
 
     def info(self):
         n, r, f, b = C.c_int32(), C.c_int32(), C.c_double(), C.c_double()

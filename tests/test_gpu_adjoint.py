@@ -46,14 +46,15 @@ def setup(case, bc):
     return gr, B, dt
 
 
-def gpu_gradient(hip, case, bc, nsteps, dt, lam, nsnap=2, tile_rows=None):
+def gpu_gradient(hip, case, bc, nsteps, dt, lam, nsnap=2, launch=None):
+    """launch: (block_waves, rows, y_rows) for HipAdjoint.set_launch, None = default."""
     c = dict(case)
     c["bc"] = bc
     with hip.HipSolver.from_case(c) as s:
         s.upload(case["h"], case["hu"], case["hv"], case["B"])
         with hip.HipAdjoint(s, nsteps, nsnap) as adj:
-            if tile_rows:
-                adj.set_launch(tile_rows, max(1, tile_rows // 2))
+            if launch:
+                adj.set_launch(*launch)
             adj.forward(dt=dt)
             final = s.download()
             grads = adj.backward(*lam)
@@ -69,28 +70,29 @@ def mild_state(nx, ny, seed, bc=0, bathy_scale=0.15):
 
 
 CASES = [
-    # (nx, ny, nsteps, bc, tile_rows, bathy_scale, rel_tol): multi-strip (ny > 58), tiles
-    # shorter than the grid, single-row tiles, tiny grids.  The last case keeps the full
-    # random bottom: nearly dry hydrostatic faces make the Jacobian huge (|J| ~ 1e40), so
-    # only its rounding-level agreement bound is looser.
+    # (nx, ny, nsteps, bc, launch, bathy_scale, rel_tol): multi-strip (ny > 58), blocks
+    # shorter than the grid, single-row waves, blocks with idle waves, tiny grids, and the
+    # two-kernel path (block_waves 0).  The last case keeps the full random bottom: nearly
+    # dry hydrostatic faces make the Jacobian huge (|J| ~ 1e40), so only its rounding-level
+    # agreement bound is looser.
     (20, 17, 1, 0, None, 0.15, 1e-10),
-    (40, 70, 3, 0, 8, 0.15, 1e-10),
-    (33, 130, 2, 1, 5, 0.15, 1e-10),
-    (70, 61, 4, 0, 16, 0.15, 1e-10),
-    (9, 7, 3, 1, 3, 0.15, 1e-10),
-    (12, 120, 2, 0, 1, 0.15, 1e-10),
-    (70, 61, 4, 0, 16, 1.0, 1e-8),
+    (40, 70, 3, 0, (8, 1, 0), 0.15, 1e-10),
+    (33, 130, 2, 1, (4, 3, 0), 0.15, 1e-10),
+    (70, 61, 4, 0, (0, 16, 8), 0.15, 1e-10),
+    (9, 7, 3, 1, (8, 4, 0), 0.15, 1e-10),
+    (12, 120, 2, 0, (4, 1, 0), 0.15, 1e-10),
+    (70, 61, 4, 0, (8, 2, 0), 1.0, 1e-8),
 ]
 
 
-@pytest.mark.parametrize("nx,ny,nsteps,bc,tile,bscale,tol", CASES)
-def test_adjoint_matches_tangent(hip, nx, ny, nsteps, bc, tile, bscale, tol):
+@pytest.mark.parametrize("nx,ny,nsteps,bc,launch,bscale,tol", CASES)
+def test_adjoint_matches_tangent(hip, nx, ny, nsteps, bc, launch, bscale, tol):
     case = mild_state(nx, ny, nx * 7 + ny, bc, bscale)
     gr, B, dt = setup(case, bc)
     rng = np.random.default_rng(nx + 100 * ny)
     shape = case["h"].shape
     lam = [interior(shape, rng) for _ in range(3)]
-    final, (gh, ghu, ghv, gb) = gpu_gradient(hip, case, bc, nsteps, dt, lam, tile_rows=tile)
+    final, (gh, ghu, ghv, gb) = gpu_gradient(hip, case, bc, nsteps, dt, lam, launch=launch)
 
     # forward under the adjoint object == the oracle's run, bitwise
     ref = [case[k].copy() for k in ("h", "hu", "hv")]
@@ -158,16 +160,32 @@ def test_gradient_independent_of_checkpoints(hip):
                 assert same(a, b), nsnap
 
 
-def test_gradient_independent_of_launch_shape(hip):
-    case = mild_state(50, 61, 12, bc=1)
-    _, _, dt = setup(case, 1)
+@pytest.mark.parametrize("bc", [0, 1])
+def test_gradient_independent_of_launch_shape(hip, bc):
+    """The fused kernel (x faces through LDS, blocks overlapping by 3 faces) and the
+    two-kernel path (x faces through device memory) sum the same terms in the same order:
+    every launch shape gives the same bits."""
+    case = mild_state(50, 61, 12, bc=bc)
+    _, _, dt = setup(case, bc)
     rng = np.random.default_rng(4)
     lam = [interior(case["h"].shape, rng) for _ in range(3)]
-    _, ref = gpu_gradient(hip, case, 1, 3, dt, lam, tile_rows=16)
-    for t in (1, 2, 7, 49, 64):
-        _, g = gpu_gradient(hip, case, 1, 3, dt, lam, tile_rows=t)
+    _, ref = gpu_gradient(hip, case, bc, 3, dt, lam, launch=(0, 16, 8))
+    for launch in ((0, 1, 1), (0, 7, 3), (0, 64, 64), (8, 1, 0), (8, 2, 0), (8, 4, 0),
+                   (4, 1, 0), (4, 3, 0), (4, 9, 0), (4, 7, 0), (2, 1, 0), (2, 5, 0)):
+        _, g = gpu_gradient(hip, case, bc, 3, dt, lam, launch=launch)
         for a, b in zip(g, ref):
-            assert same(a, b), t
+            assert same(a, b), launch
+
+
+def test_launch_shape_limits(hip):
+    case = mild_state(20, 17, 1)
+    with hip.HipSolver.from_case(case) as s, hip.HipAdjoint(s, 2, 2) as adj:
+        for bad in ((16, 2, 0), (3, 1, 0), (8, 5, 0), (4, 10, 0), (2, 19, 0)):  # W, LDS
+            with pytest.raises(hip.SweError):
+                adj.set_launch(*bad)
+        adj.set_launch(8, 4, 0)
+        adj.set_launch(4, 9, 0)
+        adj.set_launch(2, 18, 0)
 
 
 def test_adjoint_adaptive_dt_is_recorded_and_frozen(hip):
