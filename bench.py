@@ -80,8 +80,11 @@ def cpu_baseline(case, steps, dt):
 
 def ensemble_bench(steps, base_steps, check_every=32):
     """Config 4: 512 instances, sides log-uniform in [64, 1024], per-instance CFL 0.4 dt,
-    advanced together (one tile work-list launch per stage) vs one context per instance
-    (each instance's steps graph-replayed on its own).  Rates in Gcell-updates/s."""
+    advanced together (one tile work-list launch per stage) vs one context per instance.
+    Two per-instance baselines: the launch loop (every stage, speed reduction and dt
+    finalize of every instance launched on its own: 4 launches per instance and step,
+    2048 per step) and the same kernels with each instance's steps replayed from a hipGraph.
+    Rates in Gcell-updates/s."""
     import swe_hip
     from swe_hip import workloads as W
 
@@ -116,7 +119,23 @@ def ensemble_bench(steps, base_steps, check_every=32):
         s.step_adaptive(0.4, base_steps)
     el_s = time.perf_counter() - t0
     rate_s = cells * base_steps / el_s / 1e9
-    log(f"[bench] ensemble per-instance: {base_steps} steps {el_s:.3f} s -> {rate_s:.3f} Gcell/s")
+    log(f"[bench] ensemble per-instance (graph replay): {base_steps} steps {el_s:.3f} s -> "
+        f"{rate_s:.3f} Gcell/s")
+
+    # per-instance launch loop: the same kernels launched one by one
+    for s, c in zip(solvers, cases):
+        s.set_graph(False)
+        s.upload(c["h"], c["hu"], c["hv"], c["B"])
+        s.step_adaptive(0.4, 1)
+        s.upload(c["h"], c["hu"], c["hv"], c["B"])
+    solvers[-1].sync()
+    t0 = time.perf_counter()
+    for s in solvers:
+        s.step_adaptive(0.4, base_steps)
+    el_l = time.perf_counter() - t0
+    rate_l = cells * base_steps / el_l / 1e9
+    log(f"[bench] ensemble per-instance (launch loop): {base_steps} steps {el_l:.3f} s -> "
+        f"{rate_l:.3f} Gcell/s")
 
     # same number of steps in the batch from the same states; compare every check_every-th
     # instance bitwise
@@ -135,8 +154,11 @@ def ensemble_bench(steps, base_steps, check_every=32):
         "workload": "512 instances, sides log-uniform [64,1024] (seed 4), 8-bump bathymetry, "
                     "Gaussian surface, reflecting, per-instance CFL 0.4",
         "cells": cells, "batched_steps": steps, "batched_gcell_per_s": round(rate_b, 3),
-        "per_instance_steps": base_steps, "per_instance_gcell_per_s": round(rate_s, 4),
-        "speedup": round(rate_b / rate_s, 2), "tile_rows": tile_rows, "waves_per_cu": waves,
+        "per_instance_steps": base_steps,
+        "per_instance_launch_loop_gcell_per_s": round(rate_l, 4),
+        "per_instance_graph_gcell_per_s": round(rate_s, 4),
+        "speedup": round(rate_b / rate_l, 2), "speedup_vs_graph_replay": round(rate_b / rate_s, 2),
+        "tile_rows": tile_rows, "waves_per_cu": waves,
         "bitwise_vs_per_instance": bool(same),
     }
 

@@ -83,13 +83,28 @@ int blocks_for(const swe_hip_batch *b) {
 }
 
 // Work-list for the current tile_rows, instance-major so neighbouring waves share halo rows.
+// The work list: full-height tiles first (instance by instance, row segments in order),
+// then each instance's shorter last row segment, longest first, so the short tiles fill
+// the end of the launch.  Each tile computes the same values in any order.
 int build_tiles(swe_hip_batch *b) {
     b->tiles_h.clear();
+    std::vector<std::pair<int, BatchTile>> tails;  // (rows, tile)
     for (int i = 0; i < b->count; ++i) {
         const int strips = swe::strips_for(b->inst_h[i].ny);
-        for (int r0 = 2; r0 < 2 + b->inst_h[i].nx; r0 += b->tile_rows)
-            for (int s = 0; s < strips; ++s) b->tiles_h.push_back(BatchTile{i, s, r0, 0});
+        const int nx = b->inst_h[i].nx;
+        for (int r0 = 2; r0 < 2 + nx; r0 += b->tile_rows) {
+            const int rows = std::min(b->tile_rows, 2 + nx - r0);
+            for (int s = 0; s < strips; ++s) {
+                if (rows == b->tile_rows)
+                    b->tiles_h.push_back(BatchTile{i, s, r0, 0});
+                else
+                    tails.push_back({rows, BatchTile{i, s, r0, 0}});
+            }
+        }
     }
+    std::stable_sort(tails.begin(), tails.end(),
+                     [](const auto &x, const auto &y) { return x.first > y.first; });
+    for (const auto &t : tails) b->tiles_h.push_back(t.second);
     drop_graph(b);
     if (b->tiles_d) (void)hipFree(b->tiles_d);
     b->tiles_d = nullptr;
@@ -454,7 +469,7 @@ int swe_hip_batch_autotune(swe_hip_batch *b, int *tile_rows_out, int *waves_per_
     SWE_BCHECK(b, hipSetDevice(b->device));
     // Time stages 1 and 2 only: they read U and write the stage buffers, so the state the
     // caller uploaded is untouched.  dt comes from the current per-instance dt array.
-    const int tiles_c[] = {8, 16, 32, 64};
+    const int tiles_c[] = {8, 16, 32, 64, 96, 128};
     const int waves_c[] = {8, 12, 16, 24, 0};
     hipEvent_t e0, e1;
     SWE_BCHECK(b, hipEventCreate(&e0));

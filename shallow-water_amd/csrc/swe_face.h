@@ -85,8 +85,8 @@ __device__ __forceinline__ double div_numerator(double n, double d, double r) {
     return copysign(fma(rem, r, q), n);
 }
 
-// sqrt(x) for x = 0 or x >= 2^-767 (finite).
-__device__ __forceinline__ double sqrt_unscaled(double x) {
+// sqrt(x) for x >= 2^-767 (finite, non-zero).
+__device__ __forceinline__ double sqrt_positive(double x) {
     const double y = __builtin_amdgcn_rsq(x);
     double g = x * y;
     double h = y * 0.5;
@@ -96,7 +96,12 @@ __device__ __forceinline__ double sqrt_unscaled(double x) {
     double d = fma(-g, g, x);
     g = fma(d, h, g);
     d = fma(-g, g, x);
-    g = fma(d, h, g);
+    return fma(d, h, g);
+}
+
+// sqrt(x) for x = 0 or x >= 2^-767 (finite).
+__device__ __forceinline__ double sqrt_unscaled(double x) {
+    const double g = sqrt_positive(x);
     return x == 0.0 ? x : g;
 }
 
@@ -113,6 +118,151 @@ template <bool SPLIT>
 __device__ __forceinline__ double ieee_sqrt(double x) {
     if constexpr (SPLIT) return sqrt_unscaled(x);
     return sqrt(x);
+}
+
+// ---- exact mode, split forms (range guard holds) ---------------------------------------
+//
+// The same arithmetic as hll/face_flux below with SPLIT = true, organised so that
+// wave-uniform facts skip work that cannot change a result:
+//  * WET (every lane's two depths > eps): safe_velocity's selects are the identity and the
+//    sqrt arguments g*h are positive, so the zero test of the square root is dropped;
+//  * a flat bottom across the face in every lane (b- == b+): b* equals both bottoms, so the
+//    hydrostatic depths are the face depths themselves (positive_height maps a -0/+0
+//    difference to +0 either way), HLL reuses their reciprocals, and the pressure
+//    correction half_g*(h*h - hs*hs) is exactly +0 (fn + corr == fn + 0.0).
+// Each variant is a separate body, and the empty volatile asm statements around the WET
+// bodies keep the compiler from merging the two bodies back into one with selects (which
+// would cost more than the branch saves and raises register pressure).
+
+// HLL given the reciprocals rL = div_denominator(hL), rR = div_denominator(hR).
+template <bool WET>
+__device__ __forceinline__ void hll_split(double half_g, double g, double hL, double nL,
+                                          double tL, double hR, double nR, double tR, double rL,
+                                          double rR, double &fh, double &fn, double &ft) {
+    double unL = div_numerator(nL, hL, rL), utL = div_numerator(tL, hL, rL);
+    double unR = div_numerator(nR, hR, rR), utR = div_numerator(tR, hR, rR);
+    double cL, cR;
+    if constexpr (WET) {
+        cL = sqrt_positive(g * hL);
+        cR = sqrt_positive(g * hR);
+    } else {
+        unL = hL > kEps ? unL : 0.0;
+        utL = hL > kEps ? utL : 0.0;
+        unR = hR > kEps ? unR : 0.0;
+        utR = hR > kEps ? utR : 0.0;
+        cL = sqrt_unscaled(g * hL);
+        cR = sqrt_unscaled(g * hR);
+    }
+    const double sL = fmin(unL - cL, unR - cR);
+    const double sR = fmax(unL + cL, unR + cR);
+    const double FL_h = nL;
+    const double FL_n = nL * unL + half_g * hL * hL;
+    const double FL_t = nL * utL;
+    const double FR_h = nR;
+    const double FR_n = nR * unR + half_g * hR * hR;
+    const double FR_t = nR * utR;
+    auto star = [&](double &h_, double &n_, double &t_) {
+        const double d = sR - sL;
+        const double inv = div_numerator(1.0, d, div_denominator(d));
+        h_ = (sR * FL_h - sL * FR_h + sL * sR * (hR - hL)) * inv;
+        n_ = (sR * FL_n - sL * FR_n + sL * sR * (nR - nL)) * inv;
+        t_ = (sR * FL_t - sL * FR_t + sL * sR * (tR - tL)) * inv;
+    };
+    if (__all(sL < 0.0 && sR > 0.0)) {
+        star(fh, fn, ft);
+    } else if (sL >= 0.0) {
+        fh = FL_h;
+        fn = FL_n;
+        ft = FL_t;
+    } else if (sR <= 0.0) {
+        fh = FR_h;
+        fn = FR_n;
+        ft = FR_t;
+    } else {
+        star(fh, fn, ft);
+    }
+}
+
+// Face with b- == b+ in every lane; WET: every lane's face depths > eps.
+template <bool WET>
+__device__ __forceinline__ FaceFlux face_flat_split(double half_g, double g, double h_m,
+                                                    double n_m, double t_m, double h_p,
+                                                    double n_p, double t_p) {
+    const double rm = div_denominator(h_m), rp = div_denominator(h_p);
+    double un_m = div_numerator(n_m, h_m, rm), ut_m = div_numerator(t_m, h_m, rm);
+    double un_p = div_numerator(n_p, h_p, rp), ut_p = div_numerator(t_p, h_p, rp);
+    if constexpr (!WET) {
+        un_m = h_m > kEps ? un_m : 0.0;
+        ut_m = h_m > kEps ? ut_m : 0.0;
+        un_p = h_p > kEps ? un_p : 0.0;
+        ut_p = h_p > kEps ? ut_p : 0.0;
+    }
+    double fh, fn, ft;
+    hll_split<WET>(half_g, g, h_m, h_m * un_m, h_m * ut_m, h_p, h_p * un_p, h_p * ut_p, rm, rp,
+                   fh, fn, ft);
+    FaceFlux f;
+    f.fh = fh;
+    f.ft = ft;
+    f.fn_m = fn + 0.0;
+    f.fn_p = fn + 0.0;
+    return f;
+}
+
+// Face with a step in the bottom; WET: every lane's hydrostatic depths hs > eps, hence
+// also its face depths (hs = max(eta - b*, 0) <= max(eta - b, 0) = h since b* >= b).
+template <bool WET>
+__device__ __forceinline__ FaceFlux face_step_split(double half_g, double g, double h_m,
+                                                    double n_m, double t_m, double hs_m,
+                                                    double h_p, double n_p, double t_p,
+                                                    double hs_p) {
+    const double rm = div_denominator(h_m), rp = div_denominator(h_p);
+    double un_m = div_numerator(n_m, h_m, rm), ut_m = div_numerator(t_m, h_m, rm);
+    double un_p = div_numerator(n_p, h_p, rp), ut_p = div_numerator(t_p, h_p, rp);
+    if constexpr (!WET) {
+        un_m = h_m > kEps ? un_m : 0.0;
+        ut_m = h_m > kEps ? ut_m : 0.0;
+        un_p = h_p > kEps ? un_p : 0.0;
+        ut_p = h_p > kEps ? ut_p : 0.0;
+    }
+    double fh, fn, ft;
+    hll_split<WET>(half_g, g, hs_m, hs_m * un_m, hs_m * ut_m, hs_p, hs_p * un_p, hs_p * ut_p,
+                   div_denominator(hs_m), div_denominator(hs_p), fh, fn, ft);
+    const double corr_m = half_g * (h_m * h_m - hs_m * hs_m);
+    const double corr_p = half_g * (h_p * h_p - hs_p * hs_p);
+    FaceFlux f;
+    f.fh = fh;
+    f.ft = ft;
+    f.fn_m = fn + corr_m;
+    f.fn_p = fn + corr_p;
+    return f;
+}
+
+// face_flux<false, true> (below): the same values.
+__device__ __forceinline__ FaceFlux face_flux_split(double half_g, double g, double eta_m,
+                                                   double n_m, double t_m, double b_m,
+                                                   double eta_p, double n_p, double t_p,
+                                                   double b_p) {
+    const double h_m = positive_height(eta_m - b_m);
+    const double h_p = positive_height(eta_p - b_p);
+    if (__all(b_m == b_p)) {
+        if (__all(h_m > kEps && h_p > kEps)) {
+            asm volatile("");
+            const FaceFlux f = face_flat_split<true>(half_g, g, h_m, n_m, t_m, h_p, n_p, t_p);
+            asm volatile("");
+            return f;
+        }
+        return face_flat_split<false>(half_g, g, h_m, n_m, t_m, h_p, n_p, t_p);
+    }
+    const double b_star = fmax(b_m, b_p);
+    const double hs_m = positive_height(eta_m - b_star);
+    const double hs_p = positive_height(eta_p - b_star);
+    if (__all(hs_m > kEps && hs_p > kEps)) {
+        asm volatile("");
+        const FaceFlux f = face_step_split<true>(half_g, g, h_m, n_m, t_m, hs_m, h_p, n_p, t_p, hs_p);
+        asm volatile("");
+        return f;
+    }
+    return face_step_split<false>(half_g, g, h_m, n_m, t_m, hs_m, h_p, n_p, t_p, hs_p);
 }
 
 // HLL on the hydrostatically reconstructed states (omp_kernels.hpp:50-86 / :88-124).
@@ -206,6 +356,8 @@ template <bool FAST, bool SPLIT>
 __device__ __forceinline__ FaceFlux face_flux(double half_g, double g, double eta_m, double n_m,
                                               double t_m, double b_m, double eta_p, double n_p,
                                               double t_p, double b_p) {
+    if constexpr (SPLIT && !FAST)
+        return face_flux_split(half_g, g, eta_m, n_m, t_m, b_m, eta_p, n_p, t_p, b_p);
     // fmax: differs from std::max only for (b_m, b_p) = (-0, +0), where eta - b_star then
     // differs only in the sign of a zero difference, which positive_height maps to +0.
     const double b_star = fmax(b_m, b_p);
