@@ -51,6 +51,27 @@ typedef enum {
     SWE_BC_TRANSMISSIVE = 1 /* zero-gradient copy; extension, no reference counterpart */
 } swe_bc_kind;
 
+/* Scheme family of the stage kernels (swe_hip_set_scheme). */
+typedef enum {
+    SWE_SCHEME_OPTIMIZED = 0,    /* MUSCL + HLL + hydrostatic reconstruction + SSP-RK3 of the
+                                    optimized backends (omp_kernels.hpp); the default */
+    SWE_SCHEME_SERIAL = 1,       /* the modular SerialSolver with bathymetry
+                                    (flux_assembly_bathy.hpp), HLL only, MUSCL or PC */
+    SWE_SCHEME_SERIAL_PLAIN = 2  /* the modular SerialSolver without bathymetry
+                                    (flux_assembly.hpp), any reconstruction x Riemann solver */
+} swe_scheme;
+
+typedef enum {
+    SWE_RECON_MUSCL = 0,  /* reconstruction/muscl.hpp */
+    SWE_RECON_PC = 1      /* reconstruction/piecewise_const.hpp */
+} swe_recon;
+
+typedef enum {
+    SWE_RIEMANN_HLL = 0,      /* riemann/HLL.hpp */
+    SWE_RIEMANN_RUSANOV = 1,  /* riemann/rusanov.hpp */
+    SWE_RIEMANN_ROE = 2       /* riemann/ROE.hpp */
+} swe_riemann;
+
 /* Grid descriptor.  Replaces Grid (include/core/grid.hpp:3-31) + GridView
  * (omp_kernels.hpp:22-31) + constants::g (include/constants.hpp:4).  dx = Lx/Nx. */
 typedef struct {
@@ -124,6 +145,16 @@ int swe_hip_get_tile_rows(const swe_hip_ctx *ctx);
  * counterpart in the reference: FAST trades bitwise parity for fewer FP64 divisions. */
 int swe_hip_set_precision(swe_hip_ctx *ctx, int mode);
 int swe_hip_get_precision(const swe_hip_ctx *ctx);
+
+/* Scheme family of swe_hip_step / swe_hip_rhs / swe_hip_time_steps (swe_scheme).
+ * SWE_SCHEME_SERIAL* replace the modular SerialSolver<ReflectingWalls, Recon, Riemann,
+ * SSPRK3, Bathymetry> (include/backend/serial/serial_solver.hpp) for the combinations its
+ * SolverAssembly accepts (solver_assembly.hpp:315-500) and are bitwise equal to it: reflecting
+ * walls, fixed dt, no FAST precision.  recon / riemann are ignored for SWE_SCHEME_OPTIMIZED.
+ * SWE_ERR_INVALID for other combinations, a transmissive context, or while a batch/adjoint
+ * needs the optimized scheme.  With the plain scheme B is not read. */
+int swe_hip_set_scheme(swe_hip_ctx *ctx, int scheme, int recon, int riemann);
+int swe_hip_get_scheme(const swe_hip_ctx *ctx, int *scheme, int *recon, int *riemann);
 
 /* Use a caller stream (hipStream_t passed as void*; NULL = the context's own stream). */
 int swe_hip_set_stream(swe_hip_ctx *ctx, void *stream);

@@ -16,6 +16,12 @@ Two libraries:
   lie (oracle/ref_driver.cpp + ``make -C oracle ref``).  Present only where the
   reference tree was available at build time (the .so itself travels with the tree).
   g is fixed at the reference's 9.80665.
+* ``_ref/libswe_ref_modular.so`` -- likewise, the reference's modular serial solver parts
+  (reconstructions, Riemann solvers, flux assemblies, SSP-RK3; oracle/ref_modular.cpp).
+
+``liboracle.so`` also holds the restatement of the modular serial solver
+(oracle/swe_modular.c; the ``mod_*`` wrappers below), the checker of the HIP backend's
+modular scheme variants.
 
 Arrays are float64, C-contiguous, shape ``(nx + 2*ng, ny + 2*ng)`` in the reference
 layout (row-major, ``k = i*(ny+2*ng) + j``, i = x index, j = y index).
@@ -32,6 +38,11 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 TANGENT_SO = os.path.join(HERE, "libswe_tangent.so")
 REF_SO = os.path.join(HERE, "_ref", "libswe_ref.so")
+REF_MOD_SO = os.path.join(HERE, "_ref", "libswe_ref_modular.so")
+
+# modular scheme selectors (oracle/swe_modular.c; reconstruction / Riemann solver)
+RECON_MUSCL, RECON_PC = 0, 1
+RIEMANN_HLL, RIEMANN_RUSANOV, RIEMANN_ROE = 0, 1, 2
 
 G_REF = 9.80665  # include/constants.hpp:4
 BC_REFLECTING = 0
@@ -86,6 +97,18 @@ def lib():
         L.oracle_apply_bc.restype = None
         L.oracle_apply_bc.argtypes = [C.POINTER(OracleGrid), _dp, _dp, _dp]
         L.oracle_run.restype = C.c_int
+        L.oracle_mod_apply_bc.restype = None
+        L.oracle_mod_apply_bc.argtypes = [C.POINTER(OracleGrid), _dp, _dp, _dp]
+        L.oracle_mod_rhs.restype = C.c_int
+        L.oracle_mod_rhs.argtypes = [C.POINTER(OracleGrid)] + [C.c_int] * 3 + [_dp] * 7
+        L.oracle_mod_run.restype = C.c_int
+        L.oracle_mod_run.argtypes = ([C.POINTER(OracleGrid)] + [C.c_int] * 3 + [_dp] * 4 +
+                                     [C.c_int64, C.c_double])
+        L.oracle_mod_stable_dt.restype = C.c_double
+        L.oracle_mod_stable_dt.argtypes = [C.POINTER(OracleGrid), _dp, _dp, _dp, C.c_double]
+        L.oracle_mod_riemann.restype = None
+        L.oracle_mod_riemann.argtypes = [C.c_double, C.c_int, C.c_int, _dp, _dp, _dp,
+                                         C.POINTER(C.c_int)]
         L.oracle_run.argtypes = [C.POINTER(OracleGrid), _dp, _dp, _dp, _dp, C.c_int, C.c_double,
                                  C.c_long, _dp, C.POINTER(C.c_long)]
         L.oracle_num_threads.restype = C.c_int
@@ -124,6 +147,27 @@ def ref():
         R.ref_run.argtypes = grid + [_dp] * 4 + [C.c_int, C.c_double, C.c_long, _dp]
         _ref = R
     return _ref
+
+
+_refmod = None
+
+
+def ref_modular():
+    """The reference modular-solver harness, or None when it was not built."""
+    global _refmod
+    if _refmod is None and os.path.exists(REF_MOD_SO):
+        R = C.CDLL(REF_MOD_SO)
+        dims = [C.c_int, C.c_int, C.c_double, C.c_double]
+        R.ref_mod_rhs.restype = C.c_int
+        R.ref_mod_rhs.argtypes = dims + [C.c_int] * 3 + [_dp] * 7
+        R.ref_mod_run.restype = C.c_int
+        R.ref_mod_run.argtypes = dims + [C.c_int] * 3 + [_dp] * 4 + [C.c_long, C.c_double]
+        R.ref_mod_stable_dt.restype = C.c_double
+        R.ref_mod_stable_dt.argtypes = dims + [_dp] * 3 + [C.c_double]
+        R.ref_mod_riemann.restype = C.c_int
+        R.ref_mod_riemann.argtypes = [C.c_int, C.c_int, _dp, _dp, _dp]
+        _refmod = R
+    return _refmod
 
 
 def _p(a: np.ndarray):
@@ -195,6 +239,66 @@ def run(gr: OracleGrid, h, hu, hv, B, nsteps, dt=None, cfl=None):
     if rc < 0:
         raise MemoryError("oracle_run allocation failed")
     return rc, done.value, hist[:nsteps]
+
+
+# ---------------------------------------------------------------- modular serial solver
+def mod_apply_bc(gr: OracleGrid, h, hu=None, hv=None):
+    """reflecting_walls.hpp ghost fill (y walls first); hu = hv = None: scalar fill."""
+    lib().oracle_mod_apply_bc(C.byref(gr), _p(h), _p(hu) if hu is not None else None,
+                              _p(hv) if hv is not None else None)
+
+
+def mod_rhs(gr: OracleGrid, bathy, recon, riemann, h, hu, hv, B):
+    """(rc, (rh, rhu, rhv)); fills the ghosts of h, hu, hv like the reference's rhs."""
+    r = [np.zeros_like(h) for _ in range(3)]
+    Bz = B if B is not None else np.zeros_like(h)
+    rc = lib().oracle_mod_rhs(C.byref(gr), int(bathy), int(recon), int(riemann), _p(h), _p(hu),
+                              _p(hv), _p(Bz), *(_p(a) for a in r))
+    return rc, r
+
+
+def mod_run(gr: OracleGrid, bathy, recon, riemann, h, hu, hv, B, nsteps, dt):
+    """Advance in place; 0, 1 (non-finite) or 2 (HLL without dissipation speed)."""
+    Bz = B if B is not None else np.zeros_like(h)
+    return lib().oracle_mod_run(C.byref(gr), int(bathy), int(recon), int(riemann), _p(h),
+                                _p(hu), _p(hv), _p(Bz), int(nsteps), float(dt))
+
+
+def mod_stable_dt(gr: OracleGrid, h, hu, hv, cfl):
+    return lib().oracle_mod_stable_dt(C.byref(gr), _p(h), _p(hu), _p(hv), cfl)
+
+
+def mod_riemann(direction: str, kind, L, R, g=G_REF):
+    F = np.zeros(3)
+    fail = C.c_int()
+    lib().oracle_mod_riemann(g, int(direction == "y"), int(kind), _p(np.asarray(L, float)),
+                             _p(np.asarray(R, float)), _p(F), C.byref(fail))
+    return F, fail.value
+
+
+def ref_mod_rhs(nx, ny, lx, ly, bathy, recon, riemann, h, hu, hv, B):
+    r = [np.zeros_like(h) for _ in range(3)]
+    Bz = B if B is not None else np.zeros_like(h)
+    rc = ref_modular().ref_mod_rhs(nx, ny, lx, ly, int(bathy), int(recon), int(riemann), _p(h),
+                                   _p(hu), _p(hv), _p(Bz), *(_p(a) for a in r))
+    return rc, r
+
+
+def ref_mod_run(nx, ny, lx, ly, bathy, recon, riemann, h, hu, hv, B, nsteps, dt):
+    Bz = B if B is not None else np.zeros_like(h)
+    return ref_modular().ref_mod_run(nx, ny, lx, ly, int(bathy), int(recon), int(riemann), _p(h),
+                                     _p(hu), _p(hv), _p(Bz), int(nsteps), float(dt))
+
+
+def ref_mod_stable_dt(nx, ny, lx, ly, h, hu, hv, cfl):
+    return ref_modular().ref_mod_stable_dt(nx, ny, lx, ly, _p(h), _p(hu), _p(hv), cfl)
+
+
+def ref_mod_riemann(direction: str, kind, L, R):
+    F = np.zeros(3)
+    rc = ref_modular().ref_mod_riemann(int(direction == "y"), int(kind),
+                                       _p(np.asarray(L, float)), _p(np.asarray(R, float)), _p(F))
+    return F, rc
 
 
 # ---------------------------------------------------------------- reference harness

@@ -342,7 +342,7 @@ int swe_hip_adj_schedule(int64_t nsteps, int32_t nsnap, int64_t *advances, int32
 int swe_hip_adj_create(swe_hip_ctx *ctx, int64_t nsteps, int32_t nsnap, swe_hip_adj **out) {
     if (!ctx || !out || nsteps < 1 || nsnap < -1) return SWE_ERR_INVALID;
     *out = nullptr;
-    if (ctx->G.ng != 2) return SWE_ERR_INVALID;
+    if (ctx->G.ng != 2 || ctx->scheme != SWE_SCHEME_OPTIMIZED) return SWE_ERR_INVALID;
     if (hipSetDevice(ctx->device) != hipSuccess) return SWE_ERR_HIP;
     auto *a = new swe_hip_adj();
     a->ctx = ctx;

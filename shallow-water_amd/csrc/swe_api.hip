@@ -89,6 +89,18 @@ StageArgs stage_args(swe_hip_ctx *c, int stage) {
     return a;
 }
 
+int scheme_key(const swe_hip_ctx *c) { return c->scheme * 16 + c->recon * 4 + c->riemann; }
+
+// One stage launch of the context's scheme family.
+hipError_t launch_ctx_stage(swe_hip_ctx *c, int stage, bool speed, const StageArgs &a,
+                            dim3 *grid) {
+    if (c->scheme == SWE_SCHEME_OPTIMIZED)
+        return swe::launch_stage(stage, c->bc, speed, c->precision == SWE_PRECISION_FAST, c->G,
+                                 a, c->stream, grid);
+    return swe::launch_modular_stage(stage, c->scheme == SWE_SCHEME_SERIAL, c->recon, c->riemann,
+                                     c->G, a, c->stream, grid);
+}
+
 // Enqueue one step.  mode 0: fixed dt (scal[0] holds it); mode 1: adaptive, stage 3 reduces
 // the new max speed and a finalize kernel writes the next dt into scal[0] (and the history).
 int enqueue_step(swe_hip_ctx *c, int mode, double cfl) {
@@ -96,9 +108,7 @@ int enqueue_step(swe_hip_ctx *c, int mode, double cfl) {
     for (int stage = 1; stage <= 3; ++stage) {
         StageArgs a = stage_args(c, stage);
         dim3 grid;
-        SWE_CHECK(c, swe::launch_stage(stage, c->bc, speed && stage == 3,
-                                       c->precision == SWE_PRECISION_FAST, c->G, a, c->stream,
-                                       &grid));
+        SWE_CHECK(c, launch_ctx_stage(c, stage, speed && stage == 3, a, &grid));
         if (speed && stage == 3) {
             SWE_CHECK(c, swe::launch_finalize_dt(c->partial, (int)(grid.x * grid.y), cfl, c->G.dx,
                                                  c->G.dy, c->scal, c->hist, c->counter,
@@ -118,6 +128,7 @@ int run_steps(swe_hip_ctx *c, int mode, double cfl, int64_t nsteps) {
     }
     if (!c->graph || c->graph_mode != mode || c->graph_tile != c->tile_rows ||
         c->graph_precision != c->precision || c->graph_stream != c->stream ||
+        c->graph_scheme != scheme_key(c) ||
         (mode == 1 && c->graph_cfl != cfl)) {
         drop_graph(c);
         hipGraph_t g = nullptr;
@@ -137,6 +148,7 @@ int run_steps(swe_hip_ctx *c, int mode, double cfl, int64_t nsteps) {
         c->graph_mode = mode;
         c->graph_tile = c->tile_rows;
         c->graph_precision = c->precision;
+        c->graph_scheme = scheme_key(c);
         c->graph_stream = c->stream;
         c->graph_cfl = cfl;
     }
@@ -337,6 +349,8 @@ int swe_hip_step(swe_hip_ctx *c, double dt, int64_t nsteps) {
 
 int swe_hip_step_adaptive(swe_hip_ctx *c, double cfl, int64_t nsteps, double *dt_hist) {
     if (!c || nsteps < 0 || !(cfl > 0.0)) return SWE_ERR_INVALID;
+    if (c->scheme != SWE_SCHEME_OPTIMIZED)
+        return fail(c, SWE_ERR_INVALID, "the serial scheme family runs with a fixed dt");
     SWE_CHECK(c, hipSetDevice(c->device));
     const dim3 grid = swe::stage_grid(c->G, c->tile_rows);
     int rc = ensure_partial(c, (int)std::max<unsigned>(1024u, grid.x * grid.y));
@@ -371,8 +385,7 @@ int swe_hip_rhs(swe_hip_ctx *c, double *rh, double *rhu, double *rhv) {
     for (int i = 0; i < 3; ++i)
         SWE_CHECK(c, swe::launch_fill_double(c->U1[i], 0.0, (long)c->n_total, c->stream));
     StageArgs a = stage_args(c, 0);
-    SWE_CHECK(c, swe::launch_stage(0, c->bc, false, c->precision == SWE_PRECISION_FAST, c->G, a,
-                                   c->stream, nullptr));
+    SWE_CHECK(c, launch_ctx_stage(c, 0, false, a, nullptr));
     const size_t bytes = c->n_total * sizeof(double);
     SWE_CHECK(c, hipMemcpyAsync(rh, c->U1[0], bytes, hipMemcpyDeviceToHost, c->stream));
     SWE_CHECK(c, hipMemcpyAsync(rhu, c->U1[1], bytes, hipMemcpyDeviceToHost, c->stream));
@@ -403,6 +416,34 @@ int swe_hip_set_stream(swe_hip_ctx *c, void *stream) {
     return SWE_OK;
 }
 
+int swe_hip_set_scheme(swe_hip_ctx *c, int scheme, int recon, int riemann) {
+    if (!c) return SWE_ERR_INVALID;
+    if (scheme == SWE_SCHEME_OPTIMIZED) {
+        c->scheme = scheme;
+        c->recon = SWE_RECON_MUSCL;
+        c->riemann = SWE_RIEMANN_HLL;
+        return SWE_OK;
+    }
+    if (scheme != SWE_SCHEME_SERIAL && scheme != SWE_SCHEME_SERIAL_PLAIN)
+        return fail(c, SWE_ERR_INVALID, "unknown scheme");
+    if (c->bc != SWE_BC_REFLECTING)
+        return fail(c, SWE_ERR_INVALID, "the serial scheme family has reflecting walls only");
+    if (!swe::modular_supported(scheme == SWE_SCHEME_SERIAL, recon, riemann))
+        return fail(c, SWE_ERR_INVALID, "Unsupported serial solver combination");
+    c->scheme = scheme;
+    c->recon = recon;
+    c->riemann = riemann;
+    return SWE_OK;
+}
+
+int swe_hip_get_scheme(const swe_hip_ctx *c, int *scheme, int *recon, int *riemann) {
+    if (!c) return SWE_ERR_INVALID;
+    if (scheme) *scheme = c->scheme;
+    if (recon) *recon = c->recon;
+    if (riemann) *riemann = c->riemann;
+    return SWE_OK;
+}
+
 int swe_hip_set_graph(swe_hip_ctx *c, int enabled) {
     if (!c) return SWE_ERR_INVALID;
     c->use_graph = enabled != 0;
@@ -423,9 +464,7 @@ int swe_hip_time_steps(swe_hip_ctx *c, double dt, int64_t nsteps, double *stage_
     for (int64_t s = 0; s < nsteps && rc == SWE_OK; ++s) {
         for (int stage = 1; stage <= 3; ++stage) {
             StageArgs a = stage_args(c, stage);
-            const hipError_t e = swe::launch_stage(stage, c->bc, false,
-                                                   c->precision == SWE_PRECISION_FAST, c->G, a,
-                                                   c->stream, nullptr);
+            const hipError_t e = launch_ctx_stage(c, stage, false, a, nullptr);
             if (e != hipSuccess) {
                 rc = fail(c, SWE_ERR_HIP, std::string("launch_stage: ") + hipGetErrorString(e));
                 break;

@@ -29,12 +29,16 @@ struct swe_hip_ctx {
     long long hist_cap = 0;
     int tile_rows = 64;
     int precision = SWE_PRECISION_EXACT;
+    int scheme = SWE_SCHEME_OPTIMIZED;  // swe_scheme; recon / riemann for the serial family
+    int recon = SWE_RECON_MUSCL;
+    int riemann = SWE_RIEMANN_HLL;
     bool use_graph = true;
     // cached graph of one step: key = (mode, tile_rows, stream, hist pointer)
     hipGraphExec_t graph = nullptr;
     int graph_mode = -1;
     int graph_tile = -1;
     int graph_precision = -1;
+    int graph_scheme = -1;  // scheme * 16 + recon * 4 + riemann of the captured step
     hipStream_t graph_stream = nullptr;
     double graph_cfl = 0.0;
     std::string err;

@@ -44,6 +44,13 @@ hipError_t launch_stage(int stage, int bc, bool speed, bool fast, const GridDev 
                         const StageArgs &A, hipStream_t stream, dim3 *grid_out);
 dim3 stage_grid(const GridDev &G, int tile_rows);
 
+// Modular serial scheme family (swe_modular.hip): bathy = the hydrostatic flux assembly
+// (HLL only, as the reference dispatches it), else the plain one; recon 0 MUSCL / 1
+// piecewise constant; riemann 0 HLL / 1 Rusanov / 2 ROE.  Reflecting walls, fixed dt.
+bool modular_supported(bool bathy, int recon, int riemann);
+hipError_t launch_modular_stage(int stage, bool bathy, int recon, int riemann, const GridDev &G,
+                                const StageArgs &A, hipStream_t stream, dim3 *grid_out);
+
 // Max wave speed over the interior of (h, hu, hv) into partial[0..blocks), blocks <= 1024.
 hipError_t launch_speed_reduce(const GridDev &G, const double *h, const double *hu,
                                const double *hv, double *partial, int blocks,

@@ -1,0 +1,366 @@
+// swe_modular.hip -- fused SSP-RK3 stage kernels of the reference's modular serial solver
+// (SerialSolver<ReflectingWalls, Recon, Riemann, SSPRK3, Bathymetry>,
+// include/backend/serial/serial_solver.hpp) for the combinations its SolverAssembly
+// dispatches (include/solver_assembly/solver_assembly.hpp:315-500):
+//   no bathymetry  : {PiecewiseConst, MUSCL} x {HLL, Rusanov, ROE}  (flux_assembly.hpp)
+//   with bathymetry: {PiecewiseConst, MUSCL} x HLL                  (flux_assembly_bathy.hpp)
+//
+// The work decomposition is the optimized stage kernel's (swe_stage.hip): a wave owns a
+// 60-column strip plus a 2-column halo and marches down x rows with the radius-2 x stencil
+// in registers (each x face computed once), y neighbours in adjacent lanes exchanged with
+// DPP moves (each y face computed once), ghost cells synthesized by mirrored loads with
+// the reflecting walls' sign flips.  The arithmetic is the modular solver's own, operand
+// by operand, and differs from the optimized path's:
+//   * physical fluxes F (x) and G (y) of riemann.hpp:12-19 (u*u*h for x, h*v*v for y);
+//   * velocities of CellState::u/v (h > eps) inside the Riemann solvers, but
+//     flux_assembly_bathy.hpp:233-240's `h < eps ? 0 : q/h` for the hydrostatic states;
+//   * HLL divides its star flux by (sR - sL) (HLL.hpp:19); Rusanov and ROE as written;
+//   * stage combinations of ssp_rk3.hpp: U + dt L, 3/4 U + 1/4 U1 + dt/4 L,
+//     1/3 U + 2/3 U2 + (2 dt/3) L;
+//   * the positivity floor and non-finite check only with bathymetry (serial_solver.hpp:
+//     190-206); the plain path has neither.
+// Division and sqrt are the compiler's IEEE sequences (no split forms here).  Where the
+// reference's HLL throws (NaN wave speeds, HLL.hpp:23), the stage raises the non-finite
+// flag instead.
+#include "swe_face.h"
+#include "swe_internal.h"
+#include "swe_lanes.h"
+
+namespace swe {
+namespace {
+
+enum { kReconMUSCL = 0, kReconPC = 1 };
+enum { kHLL = 0, kRusanov = 1, kROE = 2 };
+
+// A state as (depth or eta, normal momentum, tangential momentum) of the face direction.
+struct NT {
+    double h, n, t;
+};
+
+// CellState::u / v (cell_state.hpp:19-26)
+__device__ __forceinline__ double cs_vel(double q, double h) { return (h > kEps) ? (q / h) : 0.0; }
+// flux_assembly_bathy.hpp:233-240
+__device__ __forceinline__ double fa_vel(double q, double h) { return (h < kEps) ? 0.0 : q / h; }
+// flux_assembly_bathy.hpp:231 std::max(0.0, h)
+__device__ __forceinline__ double fa_positive(double h) { return (0.0 < h) ? h : 0.0; }
+
+// riemann.hpp:12-19 in normal/tangential form: x uses F = (hu, u*u*h + g/2 h^2, hu*v);
+// y uses G = (hv, hu*v, h*v*v + g/2 h^2), i.e. (n, t*un, h*un*un + ...).
+template <int DIR>
+__device__ __forceinline__ NT phys(double half_g, const NT &U) {
+    const double un = cs_vel(U.n, U.h), ut = cs_vel(U.t, U.h);
+    NT f;
+    f.h = U.n;
+    if (DIR == 0) {
+        f.n = un * un * U.h + half_g * U.h * U.h;
+        f.t = U.n * ut;
+    } else {
+        f.n = U.h * un * un + half_g * U.h * U.h;
+        f.t = U.t * un;
+    }
+    return f;
+}
+
+// HLL.hpp:9-45
+template <int DIR>
+__device__ __forceinline__ NT hll_mod(double g, double half_g, const NT &L, const NT &R,
+                                      int &bad) {
+    const double uL = cs_vel(L.n, L.h), uR = cs_vel(R.n, R.h);
+    const double sR = smax(uL + sqrt(g * L.h), uR + sqrt(g * R.h));
+    const double sL = smin(uL - sqrt(g * L.h), uR - sqrt(g * R.h));
+    const NT FL = phys<DIR>(half_g, L), FR = phys<DIR>(half_g, R);
+    if (sL >= 0.0) return FL;
+    if (sL < 0.0 && sR > 0.0) {
+        const double d = sR - sL, ss = sL * sR;
+        return NT{(sR * FL.h - sL * FR.h + ss * (R.h - L.h)) / d,
+                  (sR * FL.n - sL * FR.n + ss * (R.n - L.n)) / d,
+                  (sR * FL.t - sL * FR.t + ss * (R.t - L.t)) / d};
+    }
+    if (sR <= 0.0) return FR;
+    bad = 1;  // the reference throws "HLL could not provide dissipation speed."
+    return FL;
+}
+
+// rusanov.hpp:11-30
+template <int DIR>
+__device__ __forceinline__ NT rusanov(double g, double half_g, const NT &L, const NT &R) {
+    const double uL = cs_vel(L.n, L.h), uR = cs_vel(R.n, R.h);
+    const double hL = (L.h < 0.0) ? 0.0 : L.h, hR = (R.h < 0.0) ? 0.0 : R.h;  // std::max(h, 0)
+    const double a = smax(fabs(uL) + sqrt(g * hL), fabs(uR) + sqrt(g * hR));
+    const NT FL = phys<DIR>(half_g, L), FR = phys<DIR>(half_g, R);
+    const double ha = 0.5 * a;
+    return NT{0.5 * (FL.h + FR.h) - ha * (R.h - L.h), 0.5 * (FL.n + FR.n) - ha * (R.n - L.n),
+              0.5 * (FL.t + FR.t) - ha * (R.t - L.t)};
+}
+
+// ROE.hpp:66-69
+__device__ __forceinline__ double roe_avg(double u1, double h1, double u2, double h2) {
+    return ((sqrt(h1) * u1) + (sqrt(h2) * u2)) / (sqrt(h1) + sqrt(h2));
+}
+
+// ROE.hpp:9-61.  In normal/tangential form both directions read the same: eigenvectors
+// r1 = (1, l1, ut), r2 = (0, 0, 1), r3 = (1, l3, ut); the reference's array operators
+// multiply element * scalar and add commutatively, so these sums are its sums.
+template <int DIR>
+__device__ __forceinline__ NT roe(double g, double half_g, const NT &L, const NT &R) {
+    const double h = (L.h + R.h) / 2.0;
+    const double c = sqrt(g * h);
+    const double un = roe_avg(cs_vel(L.n, L.h), L.h, cs_vel(R.n, R.h), R.h);
+    const double ut = roe_avg(cs_vel(L.t, L.h), L.h, cs_vel(R.t, R.h), R.h);
+    const double l1 = un - c, l2 = un, l3 = un + c;
+    const double dh = R.h - L.h, dn = R.n - L.n, dt = R.t - L.t;
+    const double a1 = 0.5 * (dh - ((dn - un * dh) / c));
+    const double a3 = 0.5 * (dh + ((dn - un * dh) / c));
+    const double a2 = dt - ut * dh;
+    const double s1 = fabs(l1) * a1, s2 = fabs(l2) * a2, s3 = fabs(l3) * a3;
+    const double sh = ((1.0 * s1) + (0.0 * s2)) + (1.0 * s3);
+    const double sn = ((l1 * s1) + (0.0 * s2)) + (l3 * s3);
+    const double st_ = ((ut * s1) + (1.0 * s2)) + (ut * s3);
+    const NT FL = phys<DIR>(half_g, L), FR = phys<DIR>(half_g, R);
+    return NT{0.5 * (FL.h + FR.h) - sh * 0.5, 0.5 * (FL.n + FR.n) - sn * 0.5,
+              0.5 * (FL.t + FR.t) - st_ * 0.5};
+}
+
+template <int DIR, int RIEMANN>
+__device__ __forceinline__ NT riemann(double g, double half_g, const NT &L, const NT &R,
+                                      int &bad) {
+    if constexpr (RIEMANN == kRusanov) return rusanov<DIR>(g, half_g, L, R);
+    else if constexpr (RIEMANN == kROE) return roe<DIR>(g, half_g, L, R);
+    else return hll_mod<DIR>(g, half_g, L, R, bad);
+}
+
+// One face from the reconstructed minus/plus states (w = h, or eta with bathymetry) and
+// the cell bottoms.  Returns (fh, ft, fn_minus, fn_plus) like the optimized face.
+template <int DIR, int RIEMANN, bool BATHY>
+__device__ __forceinline__ FaceFlux face_mod(double g, double half_g, const NT &m, double b_m,
+                                             const NT &p, double b_p, int &bad) {
+    FaceFlux f;
+    if constexpr (!BATHY) {
+        const NT F = riemann<DIR, RIEMANN>(g, half_g, m, p, bad);  // flux_assembly.hpp:19-25
+        f.fh = F.h;
+        f.ft = F.t;
+        f.fn_m = F.n;
+        f.fn_p = F.n;
+    } else {
+        // flux_assembly_bathy.hpp:35-88
+        const double b_star = smax(b_m, b_p);
+        const double h_m = fa_positive(m.h - b_m), h_p = fa_positive(p.h - b_p);
+        const double un_m = fa_vel(m.n, h_m), ut_m = fa_vel(m.t, h_m);
+        const double un_p = fa_vel(p.n, h_p), ut_p = fa_vel(p.t, h_p);
+        const double hs_m = fa_positive(m.h - b_star), hs_p = fa_positive(p.h - b_star);
+        const NT Um{hs_m, hs_m * un_m, hs_m * ut_m}, Up{hs_p, hs_p * un_p, hs_p * ut_p};
+        const NT F = riemann<DIR, RIEMANN>(g, half_g, Um, Up, bad);
+        f.fh = F.h;
+        f.ft = F.t;
+        f.fn_m = F.n + half_g * (h_m * h_m - hs_m * hs_m);
+        f.fn_p = F.n + half_g * (h_p * h_p - hs_p * hs_p);
+    }
+    return f;
+}
+
+struct MCell {
+    double w, hu, hv, b;  // w = h, or eta = h + B with bathymetry
+};
+
+struct MSlope {
+    double w, hu, hv;
+};
+
+__device__ __forceinline__ MSlope slope_of(const MCell &m, const MCell &c, const MCell &p) {
+    return MSlope{minmod(c.w - m.w, p.w - c.w), minmod(c.hu - m.hu, p.hu - c.hu),
+                  minmod(c.hv - m.hv, p.hv - c.hv)};
+}
+
+template <int STAGE, int RECON, int RIEMANN, bool BATHY>
+__global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, const StageArgs A) {
+    const int lane = threadIdx.x & (kWave - 1);
+    const int strip = blockIdx.x * kBlockWaves + threadIdx.x / kWave;
+    if (strip >= A.nstrips) return;
+    const int ng = G.ng, nx = G.nx, ny = G.ny, s = G.stride;
+    const int r0 = ng + (int)blockIdx.y * A.tile_rows;
+    const int r1 = (r0 + A.tile_rows < ng + nx) ? r0 + A.tile_rows : ng + nx;
+    const int rmax = ng + nx + 1;
+    const int j = ng - 2 + strip * kStripCols + lane;
+    bool gcol;
+    const int jc = mirror<0>(j < ng + ny + 2 ? j : ng + ny + 1, ng, ng + ny, gcol);
+    const unsigned jb = (unsigned)j * 8u, jcb = (unsigned)jc * 8u;
+    const bool owner = lane >= 2 && lane < kWave - 2 && j < ng + ny;
+    // the y face j+1/2 is one the reference computes (j in [ng-1, ng+ny-1],
+    // flux_assembly.hpp:33-34); x faces count in the interior columns only (:16-17)
+    const bool yface = lane >= 1 && lane < kWave - 2 && j >= ng - 1 && j <= ng + ny - 1;
+    const double g = G.g, half_g = G.half_g;
+    const double dt = STAGE != 0 ? *A.dt : 0.0;
+    int bad = 0;   // HLL without dissipation speed, or a non-finite stage output
+    int fbad = 0;  // per face, then masked
+
+    auto load = [&](int r) {
+        bool gr;
+        const int rs = mirror<0>(r, ng, ng + nx, gr);
+        const long long ro = __builtin_amdgcn_readfirstlane(rs * s);
+        MCell c;
+        const double h = ld(row_ptr(A.in_h, ro), jcb);
+        c.hu = ld(row_ptr(A.in_hu, ro), jcb);
+        c.hv = ld(row_ptr(A.in_hv, ro), jcb);
+        if (gr) c.hu = -c.hu;
+        if (gcol) c.hv = -c.hv;
+        c.b = BATHY ? ld(row_ptr(A.B, ro), jcb) : 0.0;
+        c.w = BATHY ? h + c.b : h;
+        return c;
+    };
+    // x face between rows a (minus, slope sa) and b (plus, slope sb): normal hu
+    auto xface = [&](const MCell &a, const MSlope &sa, const MCell &b, const MSlope &sb) {
+        NT m{a.w, a.hu, a.hv}, p{b.w, b.hu, b.hv};
+        if (RECON == kReconMUSCL) {
+            m = NT{a.w + 0.5 * sa.w, a.hu + 0.5 * sa.hu, a.hv + 0.5 * sa.hv};
+            p = NT{b.w - 0.5 * sb.w, b.hu - 0.5 * sb.hu, b.hv - 0.5 * sb.hv};
+        }
+        fbad = 0;
+        const FaceFlux f = face_mod<0, RIEMANN, BATHY>(g, half_g, m, a.b, p, b.b, fbad);
+        bad |= owner ? fbad : 0;
+        return f;
+    };
+    const MSlope zero{0.0, 0.0, 0.0};
+
+    // prologue: rows r0-2 .. r0+2, the plus side of face r0-1/2
+    const MCell Pm2 = load(r0 - 2), Pm1 = load(r0 - 1);
+    MCell P0 = load(r0), P1 = load(r0 + 1), P2 = load(r0 + 2 <= rmax ? r0 + 2 : rmax);
+    MSlope S0 = zero, S1 = zero;
+    if (RECON == kReconMUSCL) S0 = slope_of(Pm1, P0, P1);
+    FaceFlux Fm = xface(Pm1, RECON == kReconMUSCL ? slope_of(Pm2, Pm1, P0) : zero, P0, S0);
+
+#pragma unroll 1
+    for (int r = r0; r < r1; ++r) {
+        const MCell Pn = load(r + 3 <= rmax ? r + 3 : rmax);
+        if (RECON == kReconMUSCL) S1 = slope_of(P0, P1, P2);
+        const FaceFlux Fx = xface(P0, S0, P1, S1);
+
+        // y faces: lane j takes face j+1/2 (normal hv, tangential hu)
+        NT m{P0.w, P0.hv, P0.hu};
+        double pw = from_next(P0.w), phv = from_next(P0.hv), phu = from_next(P0.hu);
+        if (RECON == kReconMUSCL) {
+            const double dw = pw - P0.w, dv = phv - P0.hv, du = phu - P0.hu;
+            const double sw = minmod(from_prev(dw), dw), sv = minmod(from_prev(dv), dv);
+            const double su = minmod(from_prev(du), du);
+            m = NT{P0.w + 0.5 * sw, P0.hv + 0.5 * sv, P0.hu + 0.5 * su};
+            pw = from_next(P0.w - 0.5 * sw);
+            phv = from_next(P0.hv - 0.5 * sv);
+            phu = from_next(P0.hu - 0.5 * su);
+        }
+        const double pb = BATHY ? from_next(P0.b) : 0.0;
+        fbad = 0;
+        const FaceFlux Fy = face_mod<1, RIEMANN, BATHY>(g, half_g, m, P0.b, NT{pw, phv, phu}, pb,
+                                                        fbad);
+        bad |= yface ? fbad : 0;
+        const double fyp_h = from_prev(Fy.fh), fyp_hu = from_prev(Fy.ft);
+        const double fyp_hv = from_prev(Fy.fn_p);
+
+        // finite_volume(_bathy).hpp
+        const double rh = divergence(G.inv_dx, G.inv_dy, Fx.fh, Fm.fh, Fy.fh, fyp_h);
+        const double rhu = divergence(G.inv_dx, G.inv_dy, Fx.fn_m, Fm.fn_p, Fy.ft, fyp_hu);
+        const double rhv = divergence(G.inv_dx, G.inv_dy, Fx.ft, Fm.ft, Fy.fn_m, fyp_hv);
+
+        if (owner) {
+            const long long ro = __builtin_amdgcn_readfirstlane(r * s);
+            if constexpr (STAGE == 0) {
+                st(row_ptr(A.out_h, ro), jb, rh);
+                st(row_ptr(A.out_hu, ro), jb, rhu);
+                st(row_ptr(A.out_hv, ro), jb, rhv);
+            } else {
+                // the stage input is re-read (cache hit): with bathymetry P0.w is eta
+                double hN, huN, hvN;
+                if constexpr (STAGE == 1) {
+                    // ssp_rk3.hpp:10-15 (1.0 * U + dt * L)
+                    hN = 1.0 * ld(row_ptr(A.in_h, ro), jb) + dt * rh;
+                    huN = 1.0 * ld(row_ptr(A.in_hu, ro), jb) + dt * rhu;
+                    hvN = 1.0 * ld(row_ptr(A.in_hv, ro), jb) + dt * rhv;
+                } else {
+                    const double a = STAGE == 2 ? 3.0 / 4.0 : 1.0 / 3.0;
+                    const double b = STAGE == 2 ? 1.0 / 4.0 : 2.0 / 3.0;
+                    const double c = STAGE == 2 ? dt / 4.0 : 2.0 * dt / 3.0;
+                    hN = a * ld(row_ptr(A.u0_h, ro), jb) + b * ld(row_ptr(A.in_h, ro), jb) +
+                         c * rh;
+                    huN = a * ld(row_ptr(A.u0_hu, ro), jb) + b * ld(row_ptr(A.in_hu, ro), jb) +
+                          c * rhu;
+                    hvN = a * ld(row_ptr(A.u0_hv, ro), jb) + b * ld(row_ptr(A.in_hv, ro), jb) +
+                          c * rhv;
+                }
+                if constexpr (BATHY) {
+                    // serial_solver.hpp:453-472
+                    if (hN < kFloor) {
+                        hN = kFloor;
+                        huN = 0.0;
+                        hvN = 0.0;
+                    }
+                    if (!isfinite(hN) || !isfinite(huN) || !isfinite(hvN)) bad = 1;
+                }
+                st(row_ptr(A.out_h, ro), jb, hN);
+                st(row_ptr(A.out_hu, ro), jb, huN);
+                st(row_ptr(A.out_hv, ro), jb, hvN);
+            }
+        }
+        Fm = Fx;
+        P0 = P1;
+        P1 = P2;
+        P2 = Pn;
+        S0 = S1;
+    }
+    if (__any(bad)) {
+        if (lane == 0) atomicOr(A.flag, 1);
+    }
+}
+
+template <int STAGE, int RECON, int RIEMANN, bool BATHY>
+hipError_t launch_mod(const GridDev &G, const StageArgs &A, hipStream_t stream, dim3 *grid_out) {
+    const dim3 grid = stage_grid(G, A.tile_rows);
+    if (grid_out) *grid_out = grid;
+    hipLaunchKernelGGL((modular_stage_kernel<STAGE, RECON, RIEMANN, BATHY>), grid,
+                       dim3(kWave * kBlockWaves), 0, stream, G, A);
+    return hipGetLastError();
+}
+
+template <int RECON, int RIEMANN, bool BATHY>
+hipError_t launch_mod_stage(int stage, const GridDev &G, const StageArgs &A, hipStream_t stream,
+                            dim3 *grid_out) {
+    switch (stage) {
+        case 0: return launch_mod<0, RECON, RIEMANN, BATHY>(G, A, stream, grid_out);
+        case 1: return launch_mod<1, RECON, RIEMANN, BATHY>(G, A, stream, grid_out);
+        case 2: return launch_mod<2, RECON, RIEMANN, BATHY>(G, A, stream, grid_out);
+        case 3: return launch_mod<3, RECON, RIEMANN, BATHY>(G, A, stream, grid_out);
+        default: return hipErrorInvalidValue;
+    }
+}
+
+}  // namespace
+
+bool modular_supported(bool bathy, int recon, int riemann) {
+    if (recon != kReconMUSCL && recon != kReconPC) return false;
+    if (bathy) return riemann == kHLL;
+    return riemann == kHLL || riemann == kRusanov || riemann == kROE;
+}
+
+hipError_t launch_modular_stage(int stage, bool bathy, int recon, int riemann, const GridDev &G,
+                                const StageArgs &A, hipStream_t stream, dim3 *grid_out) {
+    if (G.ng != 2 || A.tile_rows < 1 || A.nstrips != strips_for(G.ny) ||
+        !modular_supported(bathy, recon, riemann))
+        return hipErrorInvalidValue;
+    if (bathy) {
+        return recon == kReconMUSCL
+                   ? launch_mod_stage<kReconMUSCL, kHLL, true>(stage, G, A, stream, grid_out)
+                   : launch_mod_stage<kReconPC, kHLL, true>(stage, G, A, stream, grid_out);
+    }
+    if (recon == kReconMUSCL) {
+        switch (riemann) {
+            case kHLL: return launch_mod_stage<kReconMUSCL, kHLL, false>(stage, G, A, stream, grid_out);
+            case kRusanov:
+                return launch_mod_stage<kReconMUSCL, kRusanov, false>(stage, G, A, stream, grid_out);
+            default: return launch_mod_stage<kReconMUSCL, kROE, false>(stage, G, A, stream, grid_out);
+        }
+    }
+    switch (riemann) {
+        case kHLL: return launch_mod_stage<kReconPC, kHLL, false>(stage, G, A, stream, grid_out);
+        case kRusanov: return launch_mod_stage<kReconPC, kRusanov, false>(stage, G, A, stream, grid_out);
+        default: return launch_mod_stage<kReconPC, kROE, false>(stage, G, A, stream, grid_out);
+    }
+}
+
+}  // namespace swe

@@ -30,6 +30,12 @@ BC_TRANSMISSIVE = 1
 PRECISION_EXACT = 0   # bitwise the reference's arithmetic
 PRECISION_FAST = 1    # reciprocal-based velocities, few-ulp differences per step
 
+SCHEME_OPTIMIZED = 0     # MUSCL + HLL + hydrostatic reconstruction (omp_kernels.hpp)
+SCHEME_SERIAL = 1        # modular SerialSolver with bathymetry (HLL; MUSCL or PC)
+SCHEME_SERIAL_PLAIN = 2  # modular SerialSolver without bathymetry (any recon x Riemann)
+RECON_MUSCL, RECON_PC = 0, 1
+RIEMANN_HLL, RIEMANN_RUSANOV, RIEMANN_ROE = 0, 1, 2
+
 DT_FIXED = 0      # one dt for every step
 DT_ADAPTIVE = 1   # dt from the CFL number at each step's start state (recorded)
 DT_GIVEN = 2      # a caller-given dt per step
@@ -56,6 +62,9 @@ class SweError(RuntimeError):
 # every symbol include/swe_hip.h declares, with (restype, argtypes)
 _SIGS = {
     "swe_hip_abi_version": (C.c_int, []),
+    "swe_hip_set_scheme": (C.c_int, [C.c_void_p, C.c_int, C.c_int, C.c_int]),
+    "swe_hip_get_scheme": (C.c_int, [C.c_void_p, C.POINTER(C.c_int), C.POINTER(C.c_int),
+                                     C.POINTER(C.c_int)]),
     "swe_hip_error_string": (C.c_char_p, [C.c_int]),
     "swe_hip_last_error": (C.c_char_p, [C.c_void_p]),
     "swe_hip_device_count": (C.c_int, []),
@@ -251,6 +260,16 @@ class HipSolver:
     @precision.setter
     def precision(self, mode):
         self._check(lib().swe_hip_set_precision(self._ctx, int(mode)))
+
+    def set_scheme(self, scheme: int, recon: int = RECON_MUSCL, riemann: int = RIEMANN_HLL):
+        """Scheme family of step/rhs (SCHEME_*): the optimized scheme, or the reference's
+        modular serial solver with the given reconstruction and Riemann solver."""
+        self._check(lib().swe_hip_set_scheme(self._ctx, int(scheme), int(recon), int(riemann)))
+
+    def get_scheme(self):
+        a, b, c = C.c_int(), C.c_int(), C.c_int()
+        self._check(lib().swe_hip_get_scheme(self._ctx, C.byref(a), C.byref(b), C.byref(c)))
+        return a.value, b.value, c.value
 
     def set_graph(self, enabled: bool):
         self._check(lib().swe_hip_set_graph(self._ctx, int(bool(enabled))))

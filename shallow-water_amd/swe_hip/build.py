@@ -23,7 +23,7 @@ REPO = os.path.dirname(PKG_ROOT)
 LIB = os.path.join(HERE, "libswe_hip.so")
 
 SOURCES = ["swe_stage.hip", "swe_api.hip", "swe_batch_api.hip", "swe_adjoint.hip",
-           "swe_adjoint_api.hip"]
+           "swe_adjoint_api.hip", "swe_modular.hip"]
 ARCH = os.environ.get("SWE_HIP_ARCH", "gfx950")
 
 HIPCC_FLAGS = [
