@@ -12,11 +12,13 @@ events recorded on the solver's stream around every stage launch in a second tim
 `cpu_baseline` times the reference's own OpenMP kernels (oracle/_ref) on the host on a
 few steps of the same grid; those steps are also compared bitwise with the GPU.
 
-Also reported: `batched` (512-instance ensemble vs per-instance launches) and `adjoint`
-(1024^2, 2000 steps: adjoint wall time over forward wall time).
+Also reported: `batched` (512-instance ensemble vs per-instance launches), `adjoint`
+(1024^2, 2000 steps: adjoint wall time over forward wall time) and `serial_schemes` (the
+reference SerialSolver's reconstruction x Riemann combinations at 4096^2, with the
+reference's own modular code timed on a 1024^2 sample).
 
     python bench.py [--steps K] [--warmup W] [--n 8192] [--cpu-steps S] [--no-cpu]
-                    [--no-ensemble] [--no-adjoint]
+                    [--no-ensemble] [--no-adjoint] [--no-serial] [--serial-n N]
 """
 from __future__ import annotations
 
@@ -210,6 +212,73 @@ def adjoint_bench(n=1024, steps=2000):
     return res
 
 
+def serial_bench(n=4096, steps=20, cpu_n=1024, cpu_steps=2):
+    """The modular serial scheme family (SWE_SCHEME_SERIAL*, the reference's SerialSolver
+    combinations) on an n^2 wet workload with bathymetry and flow (the adjoint workload's
+    generator; the plain schemes ignore B).  Per combination: fixed-dt Gcell-updates/s from
+    HIP events around every stage launch, the stage-2 kernel's algorithmic bandwidth
+    (plain: 72 B/cell, hydrostatic: 80 B/cell), and the reference's own modular code
+    (oracle/_ref, single-threaded like the reference Serial backend) on a cpu_n^2 sample of
+    the same generator, checked bitwise against the GPU on that sample."""
+    import swe_hip
+    from swe_hip import workloads as W
+    import oracle
+
+    combos = [(0, r, q) for r in (0, 1) for q in (0, 1, 2)] + [(1, 0, 0), (1, 1, 0)]
+    rn = {0: "MUSCL", 1: "PiecewiseConst"}
+    qn = {0: "HLL", 1: "Rusanov", 2: "ROE"}
+    case = W.adjoint_case(n, steps=steps)
+    small = W.adjoint_case(cpu_n, steps=cpu_steps)
+    use_ref = oracle.ref_modular() is not None
+    rows = []
+    for bathy, recon, rie in combos:
+        name = f"{'hydrostatic' if bathy else 'plain'} {rn[recon]} + {qn[rie]}"
+        with swe_hip.HipSolver.from_case(case) as s:
+            s.set_scheme(swe_hip.SCHEME_SERIAL if bathy else swe_hip.SCHEME_SERIAL_PLAIN,
+                         recon, rie)
+            s.upload(case["h"], case["hu"], case["hv"], case["B"])
+            dt = 0.2 * min(case["dx"], case["dy"])
+            s.step(dt, 2)
+            st, _ = s.time_steps(dt, steps)
+        cells = n * n
+        rate = cells / (sum(st) * 1e-3) / 1e9
+        b2 = (80 if bathy else 72) * cells
+        # CPU: the reference's modular code on the small sample, and bitwise check
+        gs = [small[k].copy() for k in ("h", "hu", "hv")]
+        dts = 0.2 * min(small["dx"], small["dy"])
+        with swe_hip.HipSolver.from_case(small) as s:
+            s.set_scheme(swe_hip.SCHEME_SERIAL if bathy else swe_hip.SCHEME_SERIAL_PLAIN,
+                         recon, rie)
+            s.upload(small["h"], small["hu"], small["hv"], small["B"])
+            s.step(dts, cpu_steps)
+            got = s.download()
+        cs = [small[k].copy() for k in ("h", "hu", "hv")]
+        t0 = time.perf_counter()
+        if use_ref:
+            rc = oracle.ref_mod_run(cpu_n, cpu_n, small["lx"], small["ly"], bathy, recon, rie,
+                                    *cs, small["B"], cpu_steps, dts)
+        else:
+            gr = oracle.grid(cpu_n, cpu_n, lx=small["lx"], ly=small["ly"])
+            rc = oracle.mod_run(gr, bathy, recon, rie, *cs, small["B"], cpu_steps, dts)
+        el = time.perf_counter() - t0
+        same = rc == 0 and all(np.array_equal(a.view(np.uint64), b.view(np.uint64))
+                               for a, b in zip(got, cs))
+        rows.append({"scheme": name, "gcell_per_s": round(rate, 3),
+                     "stage_ms": [round(x, 4) for x in st],
+                     "stage2_gbs": round(b2 / (st[1] * 1e-3) / 1e9, 1),
+                     "stage2_frac_hbm": round(b2 / (st[1] * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+                     "cpu_gcell_per_s": round(cpu_n * cpu_n * cpu_steps / el / 1e9, 5),
+                     "bitwise_vs_cpu": bool(same)})
+        log(f"[bench] serial scheme {name}: {rate:.2f} Gcell/s, CPU "
+            f"{rows[-1]['cpu_gcell_per_s']} Gcell/s, bitwise {same}")
+    return {"workload": f"{n}x{n} wet, 16-bump bathymetry + background flow, reflecting, "
+                        f"fixed dt, {steps} steps (HIP events per stage)",
+            "cpu": {"kind": "reference" if use_ref else "port", "cores": 1 if use_ref else
+                    oracle.lib().oracle_num_threads(),
+                    "sample": f"{cpu_n}x{cpu_n}, {cpu_steps} steps of the same generator"},
+            "schemes": rows}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -224,6 +293,8 @@ def main():
     ap.add_argument("--ensemble-steps", type=int, default=500)
     ap.add_argument("--ensemble-base-steps", type=int, default=30)
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-serial", action="store_true")
+    ap.add_argument("--serial-n", type=int, default=4096)
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     args = ap.parse_args()
 
@@ -331,6 +402,8 @@ def main():
         out["batched"] = ensemble_bench(args.ensemble_steps, args.ensemble_base_steps)
     if not args.no_adjoint:
         out["adjoint"] = adjoint_bench(steps=args.adjoint_steps)
+    if not args.no_serial:
+        out["serial_schemes"] = serial_bench(n=args.serial_n)
     print(json.dumps(out), flush=True)
 
 

@@ -27,7 +27,8 @@ E parse_enum(const std::string &s, const Named<E> (&names)[N], const char *what)
 const Named<BackendType> kBackends[] = {
     {"Serial", BackendType::Serial}, {"OptimizedSerial", BackendType::OptimizedSerial},
     {"OpenMP", BackendType::OpenMP}, {"CUDA", BackendType::CUDA},
-    {"CUDA_4", BackendType::CUDA_4}, {"HIP", BackendType::HIP}};
+    {"CUDA_4", BackendType::CUDA_4}, {"HIP", BackendType::HIP},
+    {"HIP_Serial", BackendType::HIP_Serial}};
 const Named<BoundaryType> kBoundaries[] = {{"ReflectingWalls", BoundaryType::ReflectingWalls}};
 const Named<ReconstructionType> kRecon[] = {{"PiecewiseConst", ReconstructionType::PiecewiseConst},
                                             {"MUSCL", ReconstructionType::MUSCL}};
@@ -165,6 +166,7 @@ std::string backend_name_from_cfg(const SimulationConfig &cfg) {
         case BackendType::CUDA: return "CUDA";
         case BackendType::CUDA_4: return "CUDA_4";
         case BackendType::HIP: return "HIP";
+        case BackendType::HIP_Serial: return "HIP_Serial";
     }
     return "UnknownBackend";
 }

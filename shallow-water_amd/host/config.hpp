@@ -9,8 +9,9 @@
 #include <filesystem>
 #include <string>
 
-// configs/config.hpp:7, plus HIP
-enum class BackendType { Serial, OptimizedSerial, OpenMP, CUDA, CUDA_4, HIP };
+// configs/config.hpp:7, plus HIP (the optimized scheme, the drop-in beside CUDA) and
+// HIP_Serial (the modular Serial backend's scheme family on the GPU)
+enum class BackendType { Serial, OptimizedSerial, OpenMP, CUDA, CUDA_4, HIP, HIP_Serial };
 
 enum class BoundaryType { ReflectingWalls };
 

@@ -1,7 +1,10 @@
 // hip_solver.cpp -- see hip_solver.hpp.
 #include "hip_solver.hpp"
 
+#include <algorithm>
 #include <chrono>
+#include <cstdio>
+#include <limits>
 #include <cmath>
 #include <iomanip>
 #include <iostream>
@@ -103,25 +106,55 @@ void warn_if_ignored_optimized_settings(const SimulationConfig &cfg, const std::
 }
 
 static std::unique_ptr<SanityCheckNetCDFWriter> make_sanity_writer(const SimulationConfig &cfg,
-                                                                   double dt) {
+                                                                   double dt,
+                                                                   const std::string &riemann,
+                                                                   const std::string &recon,
+                                                                   const std::string &bathy) {
     if (!cfg.sanity_checks.debug) return nullptr;
     if (cfg.sanity_checks.output_path.empty())
         throw std::runtime_error("sanity_checks.output_path must be set if debug mode is enabled");
     return std::make_unique<SanityCheckNetCDFWriter>(
         cfg.sanity_checks.output_path.string(), cfg.time.time_unit, cfg.mesh.spatial_unit_h,
-        cfg.time.save_every, dt, backend_name_from_cfg(cfg), "HLL", "MUSCL", "SSPRK3",
-        "Reflecting Walls", bathymetry_name_from_cfg(cfg));
+        cfg.time.save_every, dt, backend_name_from_cfg(cfg), riemann, recon, "SSPRK3",
+        "Reflecting Walls", bathy);
 }
 
-FastHLLMUSCLBathyHIPSolver::FastHLLMUSCLBathyHIPSolver(const SimulationConfig &cfg)
-    : cfg_(cfg), grid_(make_grid(cfg)),
+// serial_solver.hpp:403-418
+static std::string riemann_name_of(const HIPScheme &s) {
+    if (s.scheme == SWE_SCHEME_OPTIMIZED || s.riemann == SWE_RIEMANN_HLL) return "HLL";
+    return s.riemann == SWE_RIEMANN_RUSANOV ? "Rusanov" : "ROE";
+}
+static std::string recon_name_of(const HIPScheme &s) {
+    if (s.scheme == SWE_SCHEME_OPTIMIZED || s.recon == SWE_RECON_MUSCL) return "MUSCL";
+    return "PiecewiseConst";
+}
+
+static HIPScheme serial_scheme_of(const SimulationConfig &cfg) {
+    HIPScheme s;
+    s.scheme = cfg.bathymetry.type == BathymetryType::None ? SWE_SCHEME_SERIAL_PLAIN
+                                                           : SWE_SCHEME_SERIAL;
+    s.recon = cfg.solver.reconstruction == ReconstructionType::MUSCL ? SWE_RECON_MUSCL
+                                                                     : SWE_RECON_PC;
+    s.riemann = cfg.solver.riemann == RiemannType::HLL       ? SWE_RIEMANN_HLL
+                : cfg.solver.riemann == RiemannType::Rusanov ? SWE_RIEMANN_RUSANOV
+                                                             : SWE_RIEMANN_ROE;
+    return s;
+}
+
+SerialHIPSolver::SerialHIPSolver(const SimulationConfig &cfg)
+    : HIPSolverBase(cfg, serial_scheme_of(cfg)) {}
+
+HIPSolverBase::HIPSolverBase(const SimulationConfig &cfg, HIPScheme scheme)
+    : cfg_(cfg), scheme_(scheme), riemann_name_(riemann_name_of(scheme)),
+      recon_name_(recon_name_of(scheme)), bathy_name_(bathymetry_name_from_cfg(cfg)),
+      grid_(make_grid(cfg)),
       dt_(cfg.time.end_time / static_cast<double>(cfg.time.time_steps)),
       steps_(cfg.time.time_steps), save_every_(static_cast<std::size_t>(cfg.time.save_every)),
       writer_(cfg.output.path.string(), grid_, cfg.mesh.spatial_unit_x, cfg.mesh.spatial_unit_y,
               cfg.mesh.spatial_unit_h, cfg.time.time_unit, cfg.time.save_every),
-      sanity_checks_(make_sanity_checks(cfg)), sanity_writer_(make_sanity_writer(cfg, dt_)) {
-    if (grid_.nG < 2)
-        throw std::runtime_error("FastHLLMUSCLBathyHIPSolver requires nG >= 2 for MUSCL.");
+      sanity_checks_(make_sanity_checks(cfg)),
+      sanity_writer_(make_sanity_writer(cfg, dt_, riemann_name_, recon_name_, bathy_name_)) {
+    if (grid_.nG < 2) throw std::runtime_error("The HIP solver requires nG >= 2 for MUSCL.");
     B_ = make_bathymetry(cfg_, grid_);
     writer_.write_bathymetry(B_.data());
     make_initial_state(cfg_, grid_, h_, hu_, hv_);
@@ -136,11 +169,32 @@ FastHLLMUSCLBathyHIPSolver::FastHLLMUSCLBathyHIPSolver(const SimulationConfig &c
     d.g = kGravity;
     d.bc_kind = SWE_BC_REFLECTING;
     check(swe_hip_create(&d, cfg_.backend.device, &ctx_), "swe_hip_create");
+    if (scheme_.scheme != SWE_SCHEME_OPTIMIZED)
+        check(swe_hip_set_scheme(ctx_, scheme_.scheme, scheme_.recon, scheme_.riemann),
+              "swe_hip_set_scheme");
 }
 
-FastHLLMUSCLBathyHIPSolver::~FastHLLMUSCLBathyHIPSolver() { swe_hip_destroy(ctx_); }
+HIPSolverBase::~HIPSolverBase() { swe_hip_destroy(ctx_); }
 
-void FastHLLMUSCLBathyHIPSolver::check(int rc, const char *what) const {
+// time_utils.hpp:13-36: min over interior cells of cfl / ((|u| + c)/dx + (|v| + c)/dy), with
+// u = hu/h (no eps test); throws on h < 0.  The serial family's pre-loop CFL check.
+double HIPSolverBase::serial_stable_dt() const {
+    double lim = std::numeric_limits<double>::max();
+    for (int i = grid_.nG; i < grid_.nG + grid_.Nx; ++i)
+        for (int j = grid_.nG; j < grid_.nG + grid_.Ny; ++j) {
+            const std::size_t k = grid_.idx(i, j);
+            const double h = h_[k], hu = hu_[k], hv = hv_[k];
+            if (h < 0.0)
+                throw std::runtime_error("Could not perform stability check for dt because h <= 0!");
+            const double u = hu / h, v = hv / h;
+            const double c = cfg_.time.cfl / ((std::abs(u) + std::sqrt(kGravity * h)) / grid_.dx() +
+                                              (std::abs(v) + std::sqrt(kGravity * h)) / grid_.dy());
+            lim = std::min(lim, c);
+        }
+    return lim;
+}
+
+void HIPSolverBase::check(int rc, const char *what) const {
     if (rc == SWE_OK) return;
     std::string msg = std::string("HIP backend: ") + what + " failed: " + swe_hip_error_string(rc);
     if (ctx_) {
@@ -150,22 +204,27 @@ void FastHLLMUSCLBathyHIPSolver::check(int rc, const char *what) const {
     throw std::runtime_error(msg);
 }
 
-void FastHLLMUSCLBathyHIPSolver::download() {
+void HIPSolverBase::download() {
     const auto t0 = Clock::now();
     check(swe_hip_download(ctx_, h_.data(), hu_.data(), hv_.data()), "swe_hip_download");
     timing_.device_to_host += since(t0);
 }
 
-void FastHLLMUSCLBathyHIPSolver::run_sanity_checks(double time, std::size_t step) {
+void HIPSolverBase::run_sanity_checks(double time, std::size_t step) {
     const auto t0 = Clock::now();
     for (auto &c : sanity_checks_) c->evaluate(h_.data(), grid_, time, step, sanity_writer_.get());
     timing_.sanity_checks += since(t0);
 }
 
-void FastHLLMUSCLBathyHIPSolver::run() {
+void HIPSolverBase::run() {
     std::cout << std::unitbuf;
     std::cerr << std::unitbuf;
-    std::cout << "INFO: HIP fused-stage solver active\n";
+    const bool serial = scheme_.scheme != SWE_SCHEME_OPTIMIZED;
+    if (serial)
+        std::cout << "INFO: HIP serial-scheme solver active (" << recon_name_ << " + "
+                  << riemann_name_ << ")\n";
+    else
+        std::cout << "INFO: HIP fused-stage solver active\n";
 
     {
         const auto t0 = Clock::now();
@@ -177,7 +236,11 @@ void FastHLLMUSCLBathyHIPSolver::run() {
         // cuda_solver.hpp:159-167: the fixed dt must satisfy the CFL limit of U^0
         const auto t0 = Clock::now();
         double dt_stable = 0.0;
-        check(swe_hip_stable_dt(ctx_, cfg_.time.cfl, &dt_stable, nullptr), "swe_hip_stable_dt");
+        if (serial)
+            dt_stable = serial_stable_dt();
+        else
+            check(swe_hip_stable_dt(ctx_, cfg_.time.cfl, &dt_stable, nullptr),
+                  "swe_hip_stable_dt");
         timing_.cfl_check += since(t0);
         if (dt_ > dt_stable)
             throw std::runtime_error("Time step too large: dt = " + std::to_string(dt_) +
@@ -188,11 +251,14 @@ void FastHLLMUSCLBathyHIPSolver::run() {
     {
         const auto t0 = Clock::now();
         writer_.write_snapshot(h_.data(), hu_.data(), hv_.data(), time, dt_,
-                               backend_name_from_cfg(cfg_), "HLL", "MUSCL", "SSPRK3",
-                               "Reflecting Walls", bathymetry_name_from_cfg(cfg_));
+                               backend_name_from_cfg(cfg_), riemann_name_, recon_name_, "SSPRK3",
+                               "Reflecting Walls", bathy_name_);
         timing_.output += since(t0);
     }
     run_sanity_checks(time, 0);
+
+    // serial_solver.hpp:181
+    if (serial && scheme_.scheme == SWE_SCHEME_SERIAL) printf("\nINFO: Bathymetry enabled.\n\n");
 
     const bool estimate_eta = cfg_.output.compute_eta;
     const std::size_t eta_probe = std::min<std::size_t>(100, steps_);

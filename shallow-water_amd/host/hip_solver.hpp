@@ -1,7 +1,8 @@
-// hip_solver.hpp -- the HIP backend class, the drop-in beside the reference's
+// hip_solver.hpp -- the HIP backend classes, the drop-ins beside the reference's
 // FastHLLMUSCLBathyOpenMPSolver / FastHLLMUSCLBathyCUDASolver
-// (include/backend/cuda/cuda_solver.hpp:109-230): constructed from a SimulationConfig, run()
-// advances the configured number of steps and writes the snapshots.
+// (include/backend/cuda/cuda_solver.hpp:109-230) and its modular SerialSolver
+// (include/backend/serial/serial_solver.hpp:101-300): constructed from a SimulationConfig,
+// run() advances the configured number of steps and writes the snapshots.
 //
 // Host-only C++ (g++): the device work goes through the C ABI of libswe_hip.so
 // (include/swe_hip.h), so this file includes no HIP header.  Every failure becomes a
@@ -9,6 +10,7 @@
 #pragma once
 
 #include <memory>
+#include <string>
 #include <vector>
 
 #include "config.hpp"
@@ -29,12 +31,20 @@ struct HIPSolverTimingStats {
     void print_summary() const;
 };
 
-class FastHLLMUSCLBathyHIPSolver {
+// The scheme family a solver runs (swe_scheme / swe_recon / swe_riemann of swe_hip.h).
+struct HIPScheme {
+    int scheme = 0;  // SWE_SCHEME_OPTIMIZED
+    int recon = 0;   // SWE_RECON_MUSCL
+    int riemann = 0; // SWE_RIEMANN_HLL
+};
+
+// Shared body of the HIP solver classes below: grid, host state, output writers, sanity
+// checks and the step loop over the C ABI.
+class HIPSolverBase {
   public:
-    explicit FastHLLMUSCLBathyHIPSolver(const SimulationConfig &cfg);
-    ~FastHLLMUSCLBathyHIPSolver();
-    FastHLLMUSCLBathyHIPSolver(const FastHLLMUSCLBathyHIPSolver &) = delete;
-    FastHLLMUSCLBathyHIPSolver &operator=(const FastHLLMUSCLBathyHIPSolver &) = delete;
+    ~HIPSolverBase();
+    HIPSolverBase(const HIPSolverBase &) = delete;
+    HIPSolverBase &operator=(const HIPSolverBase &) = delete;
 
     void run();
 
@@ -43,12 +53,18 @@ class FastHLLMUSCLBathyHIPSolver {
     const std::vector<double> &hu() const { return hu_; }
     const std::vector<double> &hv() const { return hv_; }
 
+  protected:
+    HIPSolverBase(const SimulationConfig &cfg, HIPScheme scheme);
+
   private:
     void check(int rc, const char *what) const;
     void download();
     void run_sanity_checks(double time, std::size_t step);
+    double serial_stable_dt() const;
 
     const SimulationConfig cfg_;
+    const HIPScheme scheme_;
+    const std::string riemann_name_, recon_name_, bathy_name_;
     HostGrid grid_;
     double dt_;
     std::size_t steps_, save_every_;
@@ -58,6 +74,24 @@ class FastHLLMUSCLBathyHIPSolver {
     std::unique_ptr<SanityCheckNetCDFWriter> sanity_writer_;
     swe_hip_ctx *ctx_ = nullptr;
     HIPSolverTimingStats timing_;
+};
+
+// The drop-in beside FastHLLMUSCLBathyCUDASolver: the optimized MUSCL + HLL + hydrostatic
+// reconstruction scheme whatever [solver] says (solver_assembly.hpp:124-160 warns).
+class FastHLLMUSCLBathyHIPSolver : public HIPSolverBase {
+  public:
+    explicit FastHLLMUSCLBathyHIPSolver(const SimulationConfig &cfg)
+        : HIPSolverBase(cfg, HIPScheme{}) {}
+};
+
+// The reference's modular SerialSolver<ReflectingWalls, Recon, Riemann, SSPRK3, Bathymetry>
+// (include/backend/serial/serial_solver.hpp) on the GPU: reconstruction and Riemann solver
+// from [solver], the hydrostatic flux assembly with Flat / GaussHill bathymetry (HLL), the
+// plain one with bathymetry None; the serial solver's CFL-check formula
+// (time_utils.hpp:13-36) and metadata names.
+class SerialHIPSolver : public HIPSolverBase {
+  public:
+    explicit SerialHIPSolver(const SimulationConfig &cfg);
 };
 
 // solver_assembly.hpp:124-160: the fixed scheme of the optimized backends, and the config

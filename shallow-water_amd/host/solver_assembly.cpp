@@ -24,6 +24,22 @@ void SolverAssembly::run() {
                 "Unsupported HIP solver combination. The HIP backend supports MUSCL + HLL + "
                 "SSPRK3 + ReflectingWalls with Flat or GaussHill bathymetry.");
         }
+        case BackendType::HIP_Serial: {
+            // the combinations of the reference's run_serial (solver_assembly.hpp:315-500)
+            const SolverConfig &sc = cfg_.solver;
+            const bool walls = cfg_.boundary.type == BoundaryType::ReflectingWalls &&
+                               sc.time == TimeIntegratorType::SSPRK3;
+            const bool plain = cfg_.bathymetry.type == BathymetryType::None;
+            const bool bathy = (cfg_.bathymetry.type == BathymetryType::Flat ||
+                                cfg_.bathymetry.type == BathymetryType::GaussHill) &&
+                               sc.riemann == RiemannType::HLL;
+            if (walls && (plain || bathy)) {
+                SerialHIPSolver solver(cfg_);
+                solver.run();
+                return;
+            }
+            throw std::runtime_error("Unsupported serial solver combination");
+        }
         case BackendType::Serial:
         case BackendType::OptimizedSerial:
         case BackendType::OpenMP:
@@ -31,7 +47,7 @@ void SolverAssembly::run() {
         case BackendType::CUDA_4:
             throw std::runtime_error("backend type \"" + backend_name_from_cfg(cfg_) +
                                      "\" was requested, but this binary carries the HIP backend "
-                                     "only. Set [backend] type = \"HIP\".");
+                                     "only. Set [backend] type = \"HIP\" or \"HIP_Serial\".");
     }
     throw std::runtime_error("Unsupported backend");
 }

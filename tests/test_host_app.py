@@ -84,6 +84,17 @@ def test_other_backends_are_reported(tmp_path):
     assert r.returncode == 1 and "HIP backend only" in r.stderr
 
 
+def test_hip_serial_key_and_combinations(tmp_path):
+    """HIP_Serial accepts exactly the combinations of the reference's run_serial
+    (solver_assembly.hpp:315-500); the others fail before any device work."""
+    r = run("--print-config", config(tmp_path, backend__type='"HIP_Serial"'))
+    assert r.returncode == 0 and "backend.type = HIP_Serial" in r.stdout
+    for over in ({"solver__riemann": '"ROE"'},                       # bathymetry: HLL only
+                 {"solver__riemann": '"Rusanov"', "bathymetry__type": '"Flat"'}):
+        r = run(config(tmp_path, backend__type='"HIP_Serial"', **over))
+        assert r.returncode == 1 and "Unsupported serial solver combination" in r.stderr, r.stderr
+
+
 @pytest.mark.parametrize("over,msg", [
     ({"mesh__Nx": None}, "Missing or invalid TOML value for key 'mesh.Nx'"),
     ({"mesh__Nx": "40.5"}, "Missing or invalid TOML value for key 'mesh.Nx'"),
