@@ -29,7 +29,8 @@ const Named<BackendType> kBackends[] = {
     {"OpenMP", BackendType::OpenMP}, {"CUDA", BackendType::CUDA},
     {"CUDA_4", BackendType::CUDA_4}, {"HIP", BackendType::HIP},
     {"HIP_Serial", BackendType::HIP_Serial}};
-const Named<BoundaryType> kBoundaries[] = {{"ReflectingWalls", BoundaryType::ReflectingWalls}};
+const Named<BoundaryType> kBoundaries[] = {{"ReflectingWalls", BoundaryType::ReflectingWalls},
+                                           {"Transmissive", BoundaryType::Transmissive}};
 const Named<ReconstructionType> kRecon[] = {{"PiecewiseConst", ReconstructionType::PiecewiseConst},
                                             {"MUSCL", ReconstructionType::MUSCL}};
 const Named<RiemannType> kRiemann[] = {

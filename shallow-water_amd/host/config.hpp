@@ -13,7 +13,9 @@
 // HIP_Serial (the modular Serial backend's scheme family on the GPU)
 enum class BackendType { Serial, OptimizedSerial, OpenMP, CUDA, CUDA_4, HIP, HIP_Serial };
 
-enum class BoundaryType { ReflectingWalls };
+// configs/config.hpp:9, plus Transmissive (zero-gradient ghost copy; an extension honoured
+// by the HIP backend's optimized scheme)
+enum class BoundaryType { ReflectingWalls, Transmissive };
 
 enum class BathymetryType { None, Flat, GaussHill };
 

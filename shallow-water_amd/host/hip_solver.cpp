@@ -78,17 +78,13 @@ void warn_if_ignored_optimized_settings(const SimulationConfig &cfg, const std::
                   << "  reconstruction = MUSCL\n"
                   << "  riemann        = HLL\n"
                   << "  time           = SSPRK3\n"
-                  << "  boundary       = ReflectingWalls\n"
+                  << "  boundary       = ReflectingWalls or Transmissive ([boundary] type)\n"
                   << "Some [solver] settings from the config may be ignored.\n\n";
         warned = true;
     };
     if (cfg.solver.reconstruction != ReconstructionType::MUSCL) {
         begin();
         std::cerr << "  Ignored reconstruction setting from config.\n";
-    }
-    if (cfg.boundary.type != BoundaryType::ReflectingWalls) {
-        begin();
-        std::cerr << "  Ignored boundarytype setting from config.\n";
     }
     if (cfg.solver.riemann != RiemannType::HLL) {
         begin();
@@ -105,6 +101,10 @@ void warn_if_ignored_optimized_settings(const SimulationConfig &cfg, const std::
     }
 }
 
+static std::string boundary_name_of(const SimulationConfig &cfg) {
+    return cfg.boundary.type == BoundaryType::Transmissive ? "Transmissive" : "Reflecting Walls";
+}
+
 static std::unique_ptr<SanityCheckNetCDFWriter> make_sanity_writer(const SimulationConfig &cfg,
                                                                    double dt,
                                                                    const std::string &riemann,
@@ -116,7 +116,7 @@ static std::unique_ptr<SanityCheckNetCDFWriter> make_sanity_writer(const Simulat
     return std::make_unique<SanityCheckNetCDFWriter>(
         cfg.sanity_checks.output_path.string(), cfg.time.time_unit, cfg.mesh.spatial_unit_h,
         cfg.time.save_every, dt, backend_name_from_cfg(cfg), riemann, recon, "SSPRK3",
-        "Reflecting Walls", bathy);
+        boundary_name_of(cfg), bathy);
 }
 
 // serial_solver.hpp:403-418
@@ -167,7 +167,8 @@ HIPSolverBase::HIPSolverBase(const SimulationConfig &cfg, HIPScheme scheme)
     d.dx = grid_.dx();
     d.dy = grid_.dy();
     d.g = kGravity;
-    d.bc_kind = SWE_BC_REFLECTING;
+    d.bc_kind = cfg_.boundary.type == BoundaryType::Transmissive ? SWE_BC_TRANSMISSIVE
+                                                                 : SWE_BC_REFLECTING;
     check(swe_hip_create(&d, cfg_.backend.device, &ctx_), "swe_hip_create");
     if (scheme_.scheme != SWE_SCHEME_OPTIMIZED)
         check(swe_hip_set_scheme(ctx_, scheme_.scheme, scheme_.recon, scheme_.riemann),
@@ -252,7 +253,7 @@ void HIPSolverBase::run() {
         const auto t0 = Clock::now();
         writer_.write_snapshot(h_.data(), hu_.data(), hv_.data(), time, dt_,
                                backend_name_from_cfg(cfg_), riemann_name_, recon_name_, "SSPRK3",
-                               "Reflecting Walls", bathy_name_);
+                               boundary_name_of(cfg_), bathy_name_);
         timing_.output += since(t0);
     }
     run_sanity_checks(time, 0);

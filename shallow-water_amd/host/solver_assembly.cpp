@@ -22,7 +22,7 @@ void SolverAssembly::run() {
             }
             throw std::runtime_error(
                 "Unsupported HIP solver combination. The HIP backend supports MUSCL + HLL + "
-                "SSPRK3 + ReflectingWalls with Flat or GaussHill bathymetry.");
+                "SSPRK3 + ReflectingWalls or Transmissive with Flat or GaussHill bathymetry.");
         }
         case BackendType::HIP_Serial: {
             // the combinations of the reference's run_serial (solver_assembly.hpp:315-500)

@@ -113,3 +113,31 @@ def test_serial_backend_matches_modular_oracle(tmp_path, recon, riemann, bathy):
             got = d[name][rec]
             assert np.array_equal(got.view(np.uint64),
                                   np.ascontiguousarray(want[ng:-ng, ng:-ng]).view(np.uint64)), (rec, name)
+
+
+def test_transmissive_boundary_matches_oracle(tmp_path):
+    """[boundary] type = "Transmissive" with the optimized HIP scheme: zero-gradient ghosts
+    (an extension; the oracle's transmissive fill is pinned by its closed form)."""
+    over = dict(mesh__Nx=40, mesh__Ny=66, mesh__Lx=4.0, mesh__Ly=6.6, initial_condition__x0=1.0,
+                initial_condition__y0=1.5, time__end_time=0.04, time__time_steps=80,
+                time__save_every=40, boundary__type='"Transmissive"')
+    cfg = config(tmp_path, **over)
+    r = subprocess.run([APP, cfg], capture_output=True, text=True, cwd=tmp_path, timeout=120)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0
+    d, attrs = read(tmp_path / "out" / "sim.nc")
+    assert attrs["boundary_condition"] == b"Transmissive"
+    nx, ny, ng = 40, 66, 2
+    gr = oracle.grid(nx, ny, ng, 4.0 / nx, 6.6 / ny, bc=oracle.BC_TRANSMISSIVE)
+    full = lambda a: np.pad(a, ng)
+    h, hu, hv, B = (np.ascontiguousarray(full(a)) for a in
+                    (d["h"][0], d["hu"][0], d["hv"][0], d["B"]))
+    oracle.apply_bc(gr, B)
+    oracle.apply_bc(gr, h, hu, hv)
+    for rec in range(1, 3):
+        rc, _, _ = oracle.run(gr, h, hu, hv, B, 40, dt=0.04 / 80)
+        assert rc == 0
+        for name, want in (("h", h), ("hu", hu), ("hv", hv)):
+            got = d[name][rec]
+            assert np.array_equal(got.view(np.uint64),
+                                  np.ascontiguousarray(want[ng:-ng, ng:-ng]).view(np.uint64)), (rec, name)

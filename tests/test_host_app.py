@@ -90,7 +90,8 @@ def test_hip_serial_key_and_combinations(tmp_path):
     r = run("--print-config", config(tmp_path, backend__type='"HIP_Serial"'))
     assert r.returncode == 0 and "backend.type = HIP_Serial" in r.stdout
     for over in ({"solver__riemann": '"ROE"'},                       # bathymetry: HLL only
-                 {"solver__riemann": '"Rusanov"', "bathymetry__type": '"Flat"'}):
+                 {"solver__riemann": '"Rusanov"', "bathymetry__type": '"Flat"'},
+                 {"boundary__type": '"Transmissive"'}):                # reflecting walls only
         r = run(config(tmp_path, backend__type='"HIP_Serial"', **over))
         assert r.returncode == 1 and "Unsupported serial solver combination" in r.stderr, r.stderr
 
