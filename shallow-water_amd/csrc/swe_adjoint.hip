@@ -589,7 +589,7 @@ void adjoint_y_kernel(const GridDev G, const AdjArgs A) {
         const int rn = r + 1 < r1 ? r + 1 : r;
         const PCell Pn = w.load(rn, fl);
         const D3 Ln = w.load_lam(rn);
-        A4 acc = (G.split_ok && __all(Lanes<BC>::split_row(P)))
+        A4 acc = (G.split_ok && wave_all(Lanes<BC>::split_row(P)))
                      ? w.template y_row<true>(G.half_g, G.g, ky, P, L)
                      : w.template y_row<false>(G.half_g, G.g, ky, P, L);
         if (w.owner) emit_cell<STAGE, BC>(A, w, r, L, acc, gather_folded<BC>(X, G, r));
@@ -630,14 +630,14 @@ void adjoint_xface_kernel(const GridDev G, const AdjArgs A) {
     int sel0, sel1;
     {
         const PCell Pm = w.load(f0 - 1, fl);
-        split = split && __all(Lanes<BC>::split_row(Pm));
+        split = split && wave_all(Lanes<BC>::split_row(Pm));
         P0 = w.load(f0, fl);
         mark(f0, fl);
         P1 = w.load(f0 + 1, fl);
         mark(f0 + 1, fl);
         P2 = w.load(f0 + 2, fl);
         mark(f0 + 2, fl);
-        split = split && __all(Lanes<BC>::split_row(P0) && Lanes<BC>::split_row(P1) &&
+        split = split && wave_all(Lanes<BC>::split_row(P0) && Lanes<BC>::split_row(P1) &&
                                Lanes<BC>::split_row(P2));
         int sa, sb, sc;
         S0.e = half_minmod(P0.eta - Pm.eta, P1.eta - P0.eta, sa);
@@ -688,7 +688,7 @@ void adjoint_xface_kernel(const GridDev G, const AdjArgs A) {
             sr[w.j_bytes >> 3] = (unsigned char)sel0;
             if (f == ng + nx - 1) sr[w.s + (w.j_bytes >> 3)] = (unsigned char)sel1;
         }
-        split = split && __all(Lanes<BC>::split_row(P3));
+        split = split && wave_all(Lanes<BC>::split_row(P3));
         P0 = P1;
         P1 = P2;
         P2 = P3;
@@ -741,7 +741,7 @@ void adjoint_stage_kernel(const GridDev G, const AdjArgs A) {
                 P0 = w.load(fa, fl);
                 P1 = w.load(fa + 1, fl);
                 P2 = w.load(fa + 2, fl);
-                split = split && __all(Lanes<BC>::split_row(Pm) && Lanes<BC>::split_row(P0) &&
+                split = split && wave_all(Lanes<BC>::split_row(Pm) && Lanes<BC>::split_row(P0) &&
                                        Lanes<BC>::split_row(P1) && Lanes<BC>::split_row(P2));
                 int sa, sb, sc;
                 S0.e = half_minmod(P0.eta - Pm.eta, P1.eta - P0.eta, sa);
@@ -783,7 +783,7 @@ void adjoint_stage_kernel(const GridDev G, const AdjArgs A) {
                 slot[7 * kWave] = x.bp;
                 lds_sel[(f - F0) * kWave + lane] = (unsigned char)sel0;
                 if (f == F1) lds_sel[(f + 1 - F0) * kWave + lane] = (unsigned char)sel1;
-                split = split && __all(Lanes<BC>::split_row(P3));
+                split = split && wave_all(Lanes<BC>::split_row(P3));
                 P0 = P1;
                 P1 = P2;
                 P2 = P3;
@@ -810,7 +810,7 @@ void adjoint_stage_kernel(const GridDev G, const AdjArgs A) {
         const int rn = r + 1 < rb ? r + 1 : r;
         const PCell Pn = w.load(rn, fln);
         const D3 Ln = w.load_lam(rn);
-        A4 acc = (G.split_ok && __all(Lanes<BC>::split_row(P)))
+        A4 acc = (G.split_ok && wave_all(Lanes<BC>::split_row(P)))
                      ? w.template y_row<true>(G.half_g, G.g, ky, P, L)
                      : w.template y_row<false>(G.half_g, G.g, ky, P, L);
         if (w.owner) {

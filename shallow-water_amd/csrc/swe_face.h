@@ -12,6 +12,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include "swe_lanes.h"
+
 namespace swe {
 
 constexpr double kEps = 1.0e-8;    // include/constants.hpp:5 (safe_velocity threshold)
@@ -36,6 +38,18 @@ __device__ __forceinline__ double safe_velocity(double q, double h) {
 __device__ __forceinline__ double minmod(double a, double b) {
     const double r = copysign(fmin(fabs(a), fabs(b)), a);
     return (a * b <= 0.0) ? 0.0 : r;
+}
+
+// 0.5 * minmod(a, b) for finite a, b whose product does not underflow unless one of them is
+// zero.  Then minmod is the median of (a, b, 0): the smaller magnitude when the signs agree
+// and a zero otherwise, of either sign; fma(0.5, m, +0) halves exactly and turns a -0 into
+// the +0 that minmod returns.  Five instructions against the seven of 0.5 * minmod.  Under
+// the stage kernel's range guard every cell value is 0 or a multiple of 2^-292 of
+// magnitude <= 2^190, so every difference of two (eta, hu, hv) values is 0 or at least
+// 2^-292 in magnitude and the products stay far from underflow and overflow.
+__device__ __forceinline__ double half_minmod_split(double a, double b) {
+    const double m = fmax(fmin(a, b), fmin(fmax(a, b), 0.0));
+    return fma(0.5, m, 0.0);
 }
 
 // Result of one face: mass flux, tangential-momentum flux, and the normal-momentum flux as
@@ -78,11 +92,16 @@ __device__ __forceinline__ double div_denominator(double d) {
     return fma(r, e, r);
 }
 
-// n / d given r = div_denominator(d), for d > 0.
+// n / d given r = div_denominator(d), for d > 0.  The reduced sequence is
+// fma(rem, r, q) with rem = fma(-d, q, n), q = n*r, and the sign of n; here the remainder
+// is taken negated, -fma(d, q, -n): round-to-nearest is symmetric, so a non-zero remainder
+// is the same value, and an exact zero remainder comes out as -0 instead of +0.  That makes
+// the last fma return q's sign for a zero result (q = n*r with r > 0 carries the sign of n,
+// -0 included), so the sign rule needs no copysign; for a non-zero result a zero term adds
+// nothing whatever its sign.
 __device__ __forceinline__ double div_numerator(double n, double d, double r) {
     const double q = n * r;
-    const double rem = fma(-d, q, n);
-    return copysign(fma(rem, r, q), n);
+    return fma(-fma(d, q, -n), r, q);
 }
 
 // sqrt(x) for x >= 2^-767 (finite, non-zero).
@@ -138,7 +157,8 @@ __device__ __forceinline__ double ieee_sqrt(double x) {
 template <bool WET>
 __device__ __forceinline__ void hll_split(double half_g, double g, double hL, double nL,
                                           double tL, double hR, double nR, double tR, double rL,
-                                          double rR, double &fh, double &fn, double &ft) {
+                                          double rR, double &fh, double &fn, double &ft,
+                                          unsigned long long skip) {
     double unL = div_numerator(nL, hL, rL), utL = div_numerator(tL, hL, rL);
     double unR = div_numerator(nR, hR, rR), utR = div_numerator(tR, hR, rR);
     double cL, cR;
@@ -168,7 +188,7 @@ __device__ __forceinline__ void hll_split(double half_g, double g, double hL, do
         n_ = (sR * FL_n - sL * FR_n + sL * sR * (nR - nL)) * inv;
         t_ = (sR * FL_t - sL * FR_t + sL * sR * (tR - tL)) * inv;
     };
-    if (__all(sL < 0.0 && sR > 0.0)) {
+    if (wave_all(sL < 0.0 && sR > 0.0, skip)) {
         star(fh, fn, ft);
     } else if (sL >= 0.0) {
         fh = FL_h;
@@ -187,7 +207,8 @@ __device__ __forceinline__ void hll_split(double half_g, double g, double hL, do
 template <bool WET>
 __device__ __forceinline__ FaceFlux face_flat_split(double half_g, double g, double h_m,
                                                     double n_m, double t_m, double h_p,
-                                                    double n_p, double t_p) {
+                                                    double n_p, double t_p,
+                                                    unsigned long long skip) {
     const double rm = div_denominator(h_m), rp = div_denominator(h_p);
     double un_m = div_numerator(n_m, h_m, rm), ut_m = div_numerator(t_m, h_m, rm);
     double un_p = div_numerator(n_p, h_p, rp), ut_p = div_numerator(t_p, h_p, rp);
@@ -199,7 +220,7 @@ __device__ __forceinline__ FaceFlux face_flat_split(double half_g, double g, dou
     }
     double fh, fn, ft;
     hll_split<WET>(half_g, g, h_m, h_m * un_m, h_m * ut_m, h_p, h_p * un_p, h_p * ut_p, rm, rp,
-                   fh, fn, ft);
+                   fh, fn, ft, skip);
     FaceFlux f;
     f.fh = fh;
     f.ft = ft;
@@ -214,7 +235,7 @@ template <bool WET>
 __device__ __forceinline__ FaceFlux face_step_split(double half_g, double g, double h_m,
                                                     double n_m, double t_m, double hs_m,
                                                     double h_p, double n_p, double t_p,
-                                                    double hs_p) {
+                                                    double hs_p, unsigned long long skip) {
     const double rm = div_denominator(h_m), rp = div_denominator(h_p);
     double un_m = div_numerator(n_m, h_m, rm), ut_m = div_numerator(t_m, h_m, rm);
     double un_p = div_numerator(n_p, h_p, rp), ut_p = div_numerator(t_p, h_p, rp);
@@ -226,7 +247,7 @@ __device__ __forceinline__ FaceFlux face_step_split(double half_g, double g, dou
     }
     double fh, fn, ft;
     hll_split<WET>(half_g, g, hs_m, hs_m * un_m, hs_m * ut_m, hs_p, hs_p * un_p, hs_p * ut_p,
-                   div_denominator(hs_m), div_denominator(hs_p), fh, fn, ft);
+                   div_denominator(hs_m), div_denominator(hs_p), fh, fn, ft, skip);
     const double corr_m = half_g * (h_m * h_m - hs_m * hs_m);
     const double corr_p = half_g * (h_p * h_p - hs_p * hs_p);
     FaceFlux f;
@@ -241,28 +262,30 @@ __device__ __forceinline__ FaceFlux face_step_split(double half_g, double g, dou
 __device__ __forceinline__ FaceFlux face_flux_split(double half_g, double g, double eta_m,
                                                    double n_m, double t_m, double b_m,
                                                    double eta_p, double n_p, double t_p,
-                                                   double b_p) {
+                                                   double b_p, unsigned long long skip) {
     const double h_m = positive_height(eta_m - b_m);
     const double h_p = positive_height(eta_p - b_p);
-    if (__all(b_m == b_p)) {
-        if (__all(h_m > kEps && h_p > kEps)) {
+    if (wave_all(b_m == b_p, skip)) {
+        if (wave_all(h_m > kEps && h_p > kEps, skip)) {
             asm volatile("");
-            const FaceFlux f = face_flat_split<true>(half_g, g, h_m, n_m, t_m, h_p, n_p, t_p);
+            const FaceFlux f =
+                face_flat_split<true>(half_g, g, h_m, n_m, t_m, h_p, n_p, t_p, skip);
             asm volatile("");
             return f;
         }
-        return face_flat_split<false>(half_g, g, h_m, n_m, t_m, h_p, n_p, t_p);
+        return face_flat_split<false>(half_g, g, h_m, n_m, t_m, h_p, n_p, t_p, skip);
     }
     const double b_star = fmax(b_m, b_p);
     const double hs_m = positive_height(eta_m - b_star);
     const double hs_p = positive_height(eta_p - b_star);
-    if (__all(hs_m > kEps && hs_p > kEps)) {
+    if (wave_all(hs_m > kEps && hs_p > kEps, skip)) {
         asm volatile("");
-        const FaceFlux f = face_step_split<true>(half_g, g, h_m, n_m, t_m, hs_m, h_p, n_p, t_p, hs_p);
+        const FaceFlux f =
+            face_step_split<true>(half_g, g, h_m, n_m, t_m, hs_m, h_p, n_p, t_p, hs_p, skip);
         asm volatile("");
         return f;
     }
-    return face_step_split<false>(half_g, g, h_m, n_m, t_m, hs_m, h_p, n_p, t_p, hs_p);
+    return face_step_split<false>(half_g, g, h_m, n_m, t_m, hs_m, h_p, n_p, t_p, hs_p, skip);
 }
 
 // HLL on the hydrostatically reconstructed states (omp_kernels.hpp:50-86 / :88-124).
@@ -276,7 +299,7 @@ template <bool FAST, bool SPLIT>
 __device__ __forceinline__ void hll(double half_g, double g, double hL, double nL, double tL,
                                     double hR, double nR, double tR, double vnL, double vtL,
                                     double vnR, double vtR, double &fh, double &fn,
-                                    double &ft) {
+                                    double &ft, unsigned long long skip) {
     double unL, utL, unR, utR;
     if constexpr (FAST) {
         unL = hL > kEps ? vnL : 0.0;
@@ -291,7 +314,7 @@ __device__ __forceinline__ void hll(double half_g, double g, double hL, double n
         utL = hL > kEps ? qtL : 0.0;
         unR = hR > kEps ? qnR : 0.0;
         utR = hR > kEps ? qtR : 0.0;
-    } else if (__all(hL > kEps && hR > kEps)) {
+    } else if (wave_all(hL > kEps && hR > kEps, skip)) {
         // whole wave wet: safe_velocity takes its division branch in every lane
         unL = nL / hL;
         utL = tL / hL;
@@ -335,7 +358,7 @@ __device__ __forceinline__ void hll(double half_g, double g, double hL, double n
         n_ = (sR * FL_n - sL * FR_n + sL * sR * (nR - nL)) * inv;
         t_ = (sR * FL_t - sL * FR_t + sL * sR * (tR - tL)) * inv;
     };
-    if (__all(sL < 0.0 && sR > 0.0)) {
+    if (wave_all(sL < 0.0 && sR > 0.0, skip)) {
         star(fh, fn, ft);
     } else if (sL >= 0.0) {
         fh = FL_h;
@@ -352,12 +375,15 @@ __device__ __forceinline__ void hll(double half_g, double g, double hL, double n
 
 // One face from the MUSCL-reconstructed minus/plus values of (eta, n, t) and the cell
 // bottoms on either side (omp_kernels.hpp:154-189 for x, :222-257 for y).
+// skip: lanes whose inputs are not meaningful (their results are never used) and which
+// must not take part in the wave-uniform branch decisions.
 template <bool FAST, bool SPLIT>
 __device__ __forceinline__ FaceFlux face_flux(double half_g, double g, double eta_m, double n_m,
                                               double t_m, double b_m, double eta_p, double n_p,
-                                              double t_p, double b_p) {
+                                              double t_p, double b_p,
+                                              unsigned long long skip = 0ull) {
     if constexpr (SPLIT && !FAST)
-        return face_flux_split(half_g, g, eta_m, n_m, t_m, b_m, eta_p, n_p, t_p, b_p);
+        return face_flux_split(half_g, g, eta_m, n_m, t_m, b_m, eta_p, n_p, t_p, b_p, skip);
     // fmax: differs from std::max only for (b_m, b_p) = (-0, +0), where eta - b_star then
     // differs only in the sign of a zero difference, which positive_height maps to +0.
     const double b_star = fmax(b_m, b_p);
@@ -381,7 +407,7 @@ __device__ __forceinline__ FaceFlux face_flux(double half_g, double g, double et
         ut_m = h_m > kEps ? qtm : 0.0;
         un_p = h_p > kEps ? qnp : 0.0;
         ut_p = h_p > kEps ? qtp : 0.0;
-    } else if (__all(h_m > kEps && h_p > kEps)) {
+    } else if (wave_all(h_m > kEps && h_p > kEps, skip)) {
         // whole wave wet: safe_velocity takes its division branch in every lane
         un_m = n_m / h_m;
         ut_m = t_m / h_m;
@@ -399,7 +425,7 @@ __device__ __forceinline__ FaceFlux face_flux(double half_g, double g, double et
 
     double fh, fn, ft;
     hll<FAST, SPLIT>(half_g, g, hs_m, hs_m * un_m, hs_m * ut_m, hs_p, hs_p * un_p, hs_p * ut_p,
-                     un_m, ut_m, un_p, ut_p, fh, fn, ft);
+                     un_m, ut_m, un_p, ut_p, fh, fn, ft, skip);
 
     const double corr_m = half_g * (h_m * h_m - hs_m * hs_m);
     const double corr_p = half_g * (h_p * h_p - hs_p * hs_p);

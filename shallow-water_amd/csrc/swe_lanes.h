@@ -21,6 +21,34 @@ __device__ __forceinline__ double dpp_move(double x) {
 __device__ __forceinline__ double from_next(double x) { return dpp_move<0x130>(x); }
 __device__ __forceinline__ double from_prev(double x) { return dpp_move<0x138>(x); }
 
+// The same moves with bound_ctrl set: the lane without a source receives 0.  One
+// v_mov_b32_dpp per half (the keep-own-value form above first copies the source into the
+// destination, two more moves per double).  Callers must not let lane 63 (from_next_z) or
+// lane 0 (from_prev_z) reach a result, a store or a wave-wide vote (see wave_all's skip).
+template <int CTRL>
+__device__ __forceinline__ double dpp_move_z(double x) {
+    const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xF, 0xF, true);
+    const int hi = __builtin_amdgcn_mov_dpp(__double2hiint(x), CTRL, 0xF, 0xF, true);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double from_next_z(double x) { return dpp_move_z<0x130>(x); }
+__device__ __forceinline__ double from_prev_z(double x) { return dpp_move_z<0x138>(x); }
+
+// Lanes 62 and 63: the lane whose from_next_z value has no source, and the lane whose
+// y face reads that lane's slope.  Both are halo columns of the stage kernels.
+constexpr unsigned long long kLastLanes = 3ull << 62;
+
+// True when p holds in every active lane outside `skip` (wave-uniform).  The vote is a
+// ballot of the compare's lane mask against exec, all on the scalar unit; HIP's __all(int)
+// first widens the mask to a 0/1 VGPR and compares it back, two VALU instructions a vote.
+__device__ __forceinline__ bool wave_all(bool p, unsigned long long skip = 0ull) {
+    const unsigned long long exec = __builtin_amdgcn_read_exec();
+    return ((__builtin_amdgcn_ballot_w64(p) | skip) & exec) == exec;
+}
+
+// True when p holds in some active lane (wave-uniform), on the scalar unit.
+__device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
+
 // Value of lane `src` (wave-uniform) in every lane.
 __device__ __forceinline__ double read_lane(double x, int src) {
     const int lo = __builtin_amdgcn_readlane(__double2loint(x), src);

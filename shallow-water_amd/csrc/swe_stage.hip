@@ -79,7 +79,8 @@ struct March {
         const int rs = mirror<BC>(r, ng, ng + nx, gr);
         const Cell c = load_cell<BC>(A, base + __builtin_amdgcn_readfirstlane(rs * s), gr,
                                      jc_bytes, gcol);
-        split = split && __all(split_ok(c.h) && split_ok(c.hu) && split_ok(c.hv) && split_ok(c.b));
+        split = split && wave_all(split_ok(c.h) && split_ok(c.hu) && split_ok(c.hv) &&
+                                  split_ok(c.b));
         return c;
     }
 
@@ -103,34 +104,46 @@ struct March {
             row_body<false>(r, a, b, n, sa, sb, d2, d3, fxp, fxn, u0);
     }
 
+    // Half the limited slope, 0.5 * minmod(a, b) (omp_kernels.hpp:146-152, :214-220): the
+    // median form under the range guard (same bits), the reference form otherwise.
+    template <bool SPLIT>
+    static __device__ __forceinline__ double half_slope(double a, double b) {
+        if constexpr (SPLIT) return half_minmod_split(a, b);
+        return 0.5 * minmod(a, b);
+    }
+
     template <bool SPLIT>
     __device__ __forceinline__ void row_body(int r, const Cell &a, const Cell &b, const Cell &n,
                                              const V3 &sa, V3 &sb, const V3 &d2, V3 &d3,
                                              const V3 &fxp, V3 &fxn, const V3 &u0) {
         // ---- x: slope of row r+1, face r+1/2
         d3 = V3{n.eta - b.eta, n.hu - b.hu, n.hv - b.hv};
-        sb = V3{0.5 * minmod(d2.e, d3.e), 0.5 * minmod(d2.u, d3.u), 0.5 * minmod(d2.v, d3.v)};
+        sb = V3{half_slope<SPLIT>(d2.e, d3.e), half_slope<SPLIT>(d2.u, d3.u),
+                half_slope<SPLIT>(d2.v, d3.v)};
         const FaceFlux fx = face_flux<FAST, SPLIT>(half_g, g, a.eta + sa.e, a.hu + sa.u, a.hv + sa.v, a.b,
                                       b.eta - sb.e, b.hu - sb.u, b.hv - sb.v, b.b);
         fxn = V3{fx.fh, fx.fn_p, fx.ft};
 
-        // ---- y: neighbours along the lanes, face j+1/2 in lane j
-        const double dy_e = from_next(a.eta) - a.eta;
-        const double dy_u = from_next(a.hu) - a.hu;
-        const double dy_v = from_next(a.hv) - a.hv;
-        const double sy_e = 0.5 * minmod(from_prev(dy_e), dy_e);
-        const double sy_u = 0.5 * minmod(from_prev(dy_u), dy_u);
-        const double sy_v = 0.5 * minmod(from_prev(dy_v), dy_v);
-        const double pe = from_next(a.eta - sy_e);
-        const double pu = from_next(a.hu - sy_u);
-        const double pv = from_next(a.hv - sy_v);
-        const double pb = from_next(a.b);
+        // ---- y: neighbours along the lanes, face j+1/2 in lane j.  The moves fill the lane
+        // without a source with 0: lane 63's face, lane 62's (it reads lane 63's slope) and
+        // lane 0's slope and divergence belong to halo columns (never stored), and lanes 62
+        // and 63 are left out of the face's wave votes.
+        const double dy_e = from_next_z(a.eta) - a.eta;
+        const double dy_u = from_next_z(a.hu) - a.hu;
+        const double dy_v = from_next_z(a.hv) - a.hv;
+        const double sy_e = half_slope<SPLIT>(from_prev_z(dy_e), dy_e);
+        const double sy_u = half_slope<SPLIT>(from_prev_z(dy_u), dy_u);
+        const double sy_v = half_slope<SPLIT>(from_prev_z(dy_v), dy_v);
+        const double pe = from_next_z(a.eta - sy_e);
+        const double pu = from_next_z(a.hu - sy_u);
+        const double pv = from_next_z(a.hv - sy_v);
+        const double pb = from_next_z(a.b);
         // y faces: normal momentum is hv, tangential hu
         const FaceFlux fy = face_flux<FAST, SPLIT>(half_g, g, a.eta + sy_e, a.hv + sy_v, a.hu + sy_u, a.b, pe,
-                                      pv, pu, pb);
-        const double fyp_h = from_prev(fy.fh);
-        const double fyp_hu = from_prev(fy.ft);
-        const double fyp_hv = from_prev(fy.fn_p);
+                                      pv, pu, pb, kLastLanes);
+        const double fyp_h = from_prev_z(fy.fh);
+        const double fyp_hu = from_prev_z(fy.ft);
+        const double fyp_hv = from_prev_z(fy.fn_p);
 
         // ---- divergence (omp_kernels.hpp:281-283)
         const double rh = divergence(inv_dx, inv_dy, fx.fh, fxp.e, fy.fh, fyp_h);
@@ -150,13 +163,16 @@ struct March {
                 double hN = rk_combine<STAGE>(s0h, a.h, dt, rh);
                 double huN = rk_combine<STAGE>(s0u, a.hu, dt, rhu);
                 double hvN = rk_combine<STAGE>(s0v, a.hv, dt, rhv);
-                // enforce_positivity (omp_kernels.hpp:317-325)
-                if (hN < kFloor) {
-                    hN = kFloor;
-                    huN = 0.0;
-                    hvN = 0.0;
+                // enforce_positivity (omp_kernels.hpp:317-325); the selects run only when some
+                // lane of the wave is below the floor (same values)
+                if (wave_any(hN < kFloor)) {
+                    if (hN < kFloor) {
+                        hN = kFloor;
+                        huN = 0.0;
+                        hvN = 0.0;
+                    }
                 }
-                if (!isfinite(hN) || !isfinite(huN) || !isfinite(hvN)) bad = 1;
+                if (wave_any(!isfinite(hN) || !isfinite(huN) || !isfinite(hvN))) bad = 1;
                 if (SPEED) speed = cell_speed(g, hN, huN, hvN, speed);
                 st(row_ptr(A.out_h, ro), j_bytes, hN);
                 st(row_ptr(A.out_hu, ro), j_bytes, huN);
@@ -167,10 +183,15 @@ struct March {
 };
 
 // Occupancy target of the stage kernel (waves per SIMD); the VGPR budget follows from it
-// (512 / waves).  Measured on the 8192^2 dam break: 3 waves (134 VGPRs, no spills) beats
-// 4 (128 VGPRs) by ~2%; 5 and 6 spill.  Overridable at build time for tuning sweeps.
+// (512 / waves, rounded down to the allocation granule: 168 at 3 waves).  Measured on the
+// 8192^2 dam break: 3 waves (no spills) beats 4 (128 VGPRs) by ~2%; 5 and 6 spill.  The
+// persistent batch kernel has its own target.  Both are overridable at build time for
+// tuning sweeps.
 #ifndef SWE_STAGE_WAVES_PER_EU
 #define SWE_STAGE_WAVES_PER_EU 3
+#endif
+#ifndef SWE_BATCH_WAVES_PER_EU
+#define SWE_BATCH_WAVES_PER_EU SWE_STAGE_WAVES_PER_EU
 #endif
 
 // One wave's tile: strip `strip` (60 output columns) of rows [r0, min(r0 + tile_rows,
@@ -277,7 +298,7 @@ __device__ __forceinline__ double wave_max(double v) {
 // speed goes to max_bits[inst] with a 64-bit atomic max on the bit pattern (exact for
 // non-negative doubles, order-independent); non-finite flags per instance.
 template <int STAGE, int BC, bool SPEED, bool FAST>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SWE_STAGE_WAVES_PER_EU)))
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SWE_BATCH_WAVES_PER_EU)))
 void batch_stage_kernel(const BatchArgs B) {
     const int lane = threadIdx.x & (kWave - 1);
     const int gw = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave));
