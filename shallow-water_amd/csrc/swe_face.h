@@ -47,8 +47,23 @@ __device__ __forceinline__ double minmod(double a, double b) {
 // the stage kernel's range guard every cell value is 0 or a multiple of 2^-292 of
 // magnitude <= 2^190, so every difference of two (eta, hu, hv) values is 0 or at least
 // 2^-292 in magnitude and the products stay far from underflow and overflow.
+// The min/max are the plain instructions: LLVM's fmin/fmax in IEEE mode first quiet a
+// signalling NaN on every operand it cannot prove canonical (DPP moves and loop-carried
+// values here), one extra v_max_f64 each; the operands are finite under the guard.  The
+// results feed only the fma below (never a DPP move, so no VALU-to-DPP hazard is hidden
+// from the compiler).
+__device__ __forceinline__ double vmin_f64(double a, double b) {
+    double r;
+    asm("v_min_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
+__device__ __forceinline__ double vmax_f64(double a, double b) {
+    double r;
+    asm("v_max_f64 %0, %1, %2" : "=v"(r) : "v"(a), "v"(b));
+    return r;
+}
 __device__ __forceinline__ double half_minmod_split(double a, double b) {
-    const double m = fmax(fmin(a, b), fmin(fmax(a, b), 0.0));
+    const double m = vmax_f64(vmin_f64(a, b), vmin_f64(vmax_f64(a, b), 0.0));
     return fma(0.5, m, 0.0);
 }
 
@@ -131,6 +146,14 @@ __device__ __forceinline__ bool split_ok(double v) {
     return (a >= 0x1p-240 && a <= 0x1p190) || v == 0.0;
 }
 
+// The lanes of the wave where split_ok(v) holds, one ballot per compare (the masks combine
+// on the scalar unit instead of as 0/1 vector values).
+__device__ __forceinline__ unsigned long long split_ok_lanes(double v) {
+    const double a = fabs(v);
+    return (__builtin_amdgcn_ballot_w64(a >= 0x1p-240) & __builtin_amdgcn_ballot_w64(a <= 0x1p190)) |
+           __builtin_amdgcn_ballot_w64(v == 0.0);
+}
+
 // Division and square root in the exact mode: the split forms when SPLIT (range proven),
 // the compiler's full sequences otherwise.
 template <bool SPLIT>
@@ -188,7 +211,7 @@ __device__ __forceinline__ void hll_split(double half_g, double g, double hL, do
         n_ = (sR * FL_n - sL * FR_n + sL * sR * (nR - nL)) * inv;
         t_ = (sR * FL_t - sL * FR_t + sL * sR * (tR - tL)) * inv;
     };
-    if (wave_all(sL < 0.0 && sR > 0.0, skip)) {
+    if (wave_all2(sL < 0.0, sR > 0.0, skip)) {
         star(fh, fn, ft);
     } else if (sL >= 0.0) {
         fh = FL_h;
@@ -266,7 +289,7 @@ __device__ __forceinline__ FaceFlux face_flux_split(double half_g, double g, dou
     const double h_m = positive_height(eta_m - b_m);
     const double h_p = positive_height(eta_p - b_p);
     if (wave_all(b_m == b_p, skip)) {
-        if (wave_all(h_m > kEps && h_p > kEps, skip)) {
+        if (wave_all2(h_m > kEps, h_p > kEps, skip)) {
             asm volatile("");
             const FaceFlux f =
                 face_flat_split<true>(half_g, g, h_m, n_m, t_m, h_p, n_p, t_p, skip);
@@ -278,7 +301,7 @@ __device__ __forceinline__ FaceFlux face_flux_split(double half_g, double g, dou
     const double b_star = fmax(b_m, b_p);
     const double hs_m = positive_height(eta_m - b_star);
     const double hs_p = positive_height(eta_p - b_star);
-    if (wave_all(hs_m > kEps && hs_p > kEps, skip)) {
+    if (wave_all2(hs_m > kEps, hs_p > kEps, skip)) {
         asm volatile("");
         const FaceFlux f =
             face_step_split<true>(half_g, g, h_m, n_m, t_m, hs_m, h_p, n_p, t_p, hs_p, skip);
@@ -314,7 +337,7 @@ __device__ __forceinline__ void hll(double half_g, double g, double hL, double n
         utL = hL > kEps ? qtL : 0.0;
         unR = hR > kEps ? qnR : 0.0;
         utR = hR > kEps ? qtR : 0.0;
-    } else if (wave_all(hL > kEps && hR > kEps, skip)) {
+    } else if (wave_all2(hL > kEps, hR > kEps, skip)) {
         // whole wave wet: safe_velocity takes its division branch in every lane
         unL = nL / hL;
         utL = tL / hL;
@@ -358,7 +381,7 @@ __device__ __forceinline__ void hll(double half_g, double g, double hL, double n
         n_ = (sR * FL_n - sL * FR_n + sL * sR * (nR - nL)) * inv;
         t_ = (sR * FL_t - sL * FR_t + sL * sR * (tR - tL)) * inv;
     };
-    if (wave_all(sL < 0.0 && sR > 0.0, skip)) {
+    if (wave_all2(sL < 0.0, sR > 0.0, skip)) {
         star(fh, fn, ft);
     } else if (sL >= 0.0) {
         fh = FL_h;
@@ -407,7 +430,7 @@ __device__ __forceinline__ FaceFlux face_flux(double half_g, double g, double et
         ut_m = h_m > kEps ? qtm : 0.0;
         un_p = h_p > kEps ? qnp : 0.0;
         ut_p = h_p > kEps ? qtp : 0.0;
-    } else if (wave_all(h_m > kEps && h_p > kEps, skip)) {
+    } else if (wave_all2(h_m > kEps, h_p > kEps, skip)) {
         // whole wave wet: safe_velocity takes its division branch in every lane
         un_m = n_m / h_m;
         ut_m = t_m / h_m;

@@ -49,6 +49,20 @@ __device__ __forceinline__ bool wave_all(bool p, unsigned long long skip = 0ull)
 // True when p holds in some active lane (wave-uniform), on the scalar unit.
 __device__ __forceinline__ bool wave_any(bool p) { return __builtin_amdgcn_ballot_w64(p) != 0; }
 
+// wave_all(p && q, skip) with one ballot per compare, combined on the scalar unit (LLVM
+// otherwise materializes the && of two compares as a 0/1 VGPR before the vote).
+__device__ __forceinline__ bool wave_all2(bool p, bool q, unsigned long long skip = 0ull) {
+    const unsigned long long exec = __builtin_amdgcn_read_exec();
+    return (((__builtin_amdgcn_ballot_w64(p) & __builtin_amdgcn_ballot_w64(q)) | skip) & exec) ==
+           exec;
+}
+
+// wave_any(p || q || r), one ballot per compare.
+__device__ __forceinline__ bool wave_any3(bool p, bool q, bool r) {
+    return (__builtin_amdgcn_ballot_w64(p) | __builtin_amdgcn_ballot_w64(q) |
+            __builtin_amdgcn_ballot_w64(r)) != 0;
+}
+
 // Value of lane `src` (wave-uniform) in every lane.
 __device__ __forceinline__ double read_lane(double x, int src) {
     const int lo = __builtin_amdgcn_readlane(__double2loint(x), src);
@@ -76,18 +90,37 @@ __device__ __forceinline__ int mirror(int r, int lo, int hi, bool &ghost) {
 // The row offset is wave-uniform and the column is fixed per lane, so every access is a
 // scalar row base plus a constant 32-bit per-lane byte offset: the global_load/store
 // "saddr + voffset" form, with no per-row 64-bit address arithmetic on the vector ALUs.
-// row_off = instance offset (0 for a single grid) + row * stride, wave-uniform.
-__device__ __forceinline__ const double *row_ptr(const double *base, long long row_off) {
-    return base + row_off;
+// row_off = instance offset (0 for a single grid) + row * stride, wave-uniform.  The row
+// base passes through an empty asm statement that pins it to scalar registers: without
+// it LLVM reassociates (base + row) + lane into a hoisted per-lane 64-bit pointer
+// (two VGPRs per array) plus a 64-bit vector add per access.
+__device__ __forceinline__ unsigned long long uniform_addr(unsigned long long p) {
+    asm("" : "+s"(p));
+    return p;
 }
-__device__ __forceinline__ double *row_ptr(double *base, long long row_off) {
-    return base + row_off;
+// Pointers into global memory (address space 1): loads and stores through them are
+// global_* instructions, which have the saddr form (flat_* ones do not).
+using GConstRow = const __attribute__((address_space(1))) char *;
+using GRow = __attribute__((address_space(1))) char *;
+__device__ __forceinline__ GConstRow row_ptr(const double *base, long long row_off) {
+    return (GConstRow)uniform_addr(reinterpret_cast<unsigned long long>(base + row_off));
 }
-__device__ __forceinline__ double ld(const double *row, unsigned byte_off) {
-    return *reinterpret_cast<const double *>(reinterpret_cast<const char *>(row) + byte_off);
+__device__ __forceinline__ GRow row_ptr(double *base, long long row_off) {
+    return (GRow)uniform_addr(reinterpret_cast<unsigned long long>(base + row_off));
 }
-__device__ __forceinline__ void st(double *row, unsigned byte_off, double v) {
-    *reinterpret_cast<double *>(reinterpret_cast<char *>(row) + byte_off) = v;
+// Re-define a per-lane byte offset in place at row r, with no instruction.  The offset
+// stays loop-variant, so LLVM does not hoist its zero extension out of the row loop: a
+// hoisted 64-bit extension reaches the loop's loads and stores as a plain register pair,
+// which rules out the saddr + voffset form (one 64-bit vector add per access instead).
+// Call it in the basic block of the accesses that use the offset.
+__device__ __forceinline__ void pin_lane_offset(unsigned &off, int r) {
+    asm("" : "+v"(off) : "s"(r));
+}
+__device__ __forceinline__ double ld(GConstRow row, unsigned byte_off) {
+    return *(const __attribute__((address_space(1))) double *)(row + byte_off);
+}
+__device__ __forceinline__ void st(GRow row, unsigned byte_off, double v) {
+    *(__attribute__((address_space(1))) double *)(row + byte_off) = v;
 }
 
 }  // namespace swe

@@ -75,16 +75,19 @@ struct March {
     bool split;
 
     __device__ __forceinline__ Cell load(int r) {
+        pin_lane_offset(jc_bytes, r);
         bool gr;
         const int rs = mirror<BC>(r, ng, ng + nx, gr);
         const Cell c = load_cell<BC>(A, base + __builtin_amdgcn_readfirstlane(rs * s), gr,
                                      jc_bytes, gcol);
-        split = split && wave_all(split_ok(c.h) && split_ok(c.hu) && split_ok(c.hv) &&
-                                  split_ok(c.b));
+        const unsigned long long exec = __builtin_amdgcn_read_exec();
+        split = split && (split_ok_lanes(c.h) & split_ok_lanes(c.hu) & split_ok_lanes(c.hv) &
+                          split_ok_lanes(c.b) & exec) == exec;
         return c;
     }
 
-    __device__ __forceinline__ V3 load_u0(int r) const {
+    __device__ __forceinline__ V3 load_u0(int r) {
+        pin_lane_offset(jc_bytes, r);
         const long long ro = base + __builtin_amdgcn_readfirstlane(r * s);
         return V3{ld(row_ptr(A.u0_h, ro), jc_bytes), ld(row_ptr(A.u0_hu, ro), jc_bytes),
                   ld(row_ptr(A.u0_hv, ro), jc_bytes)};
@@ -153,6 +156,7 @@ struct March {
         if (owner) {
             const long long ro = base + __builtin_amdgcn_readfirstlane(r * s);
             if (STAGE == 0) {
+                pin_lane_offset(j_bytes, r);
                 st(row_ptr(A.out_h, ro), j_bytes, rh);
                 st(row_ptr(A.out_hu, ro), j_bytes, rhu);
                 st(row_ptr(A.out_hv, ro), j_bytes, rhv);
@@ -172,8 +176,9 @@ struct March {
                         hvN = 0.0;
                     }
                 }
-                if (wave_any(!isfinite(hN) || !isfinite(huN) || !isfinite(hvN))) bad = 1;
+                if (wave_any3(!isfinite(hN), !isfinite(huN), !isfinite(hvN))) bad = 1;
                 if (SPEED) speed = cell_speed(g, hN, huN, hvN, speed);
+                pin_lane_offset(j_bytes, r);
                 st(row_ptr(A.out_h, ro), j_bytes, hN);
                 st(row_ptr(A.out_hu, ro), j_bytes, huN);
                 st(row_ptr(A.out_hv, ro), j_bytes, hvN);
