@@ -7,8 +7,9 @@
 //
 // The work decomposition is the optimized stage kernel's (swe_stage.hip): a wave owns a
 // 60-column strip plus a 2-column halo and marches down x rows with the radius-2 x stencil
-// in registers (each x face computed once), y neighbours in adjacent lanes exchanged with
-// DPP moves (each y face computed once), ghost cells synthesized by mirrored loads with
+// in registers (each x face computed once, the march unrolled 4x with the register window
+// renamed), y neighbours in adjacent lanes exchanged with DPP moves (each y face computed
+// once), ghost cells synthesized by mirrored loads with
 // the reflecting walls' sign flips.  The arithmetic is the modular solver's own, operand
 // by operand, and differs from the optimized path's:
 //   * physical fluxes F (x) and G (y) of riemann.hpp:12-19 (u*u*h for x, h*v*v for y);
@@ -183,7 +184,7 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
     const int j = ng - 2 + strip * kStripCols + lane;
     bool gcol;
     const int jc = mirror<0>(j < ng + ny + 2 ? j : ng + ny + 1, ng, ng + ny, gcol);
-    const unsigned jb = (unsigned)j * 8u, jcb = (unsigned)jc * 8u;
+    unsigned jb = (unsigned)j * 8u, jcb = (unsigned)jc * 8u;
     const bool owner = lane >= 2 && lane < kWave - 2 && j < ng + ny;
     // the y face j+1/2 is one the reference computes (j in [ng-1, ng+ny-1],
     // flux_assembly.hpp:33-34); x faces count in the interior columns only (:16-17)
@@ -194,6 +195,7 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
     int fbad = 0;  // per face, then masked
 
     auto load = [&](int r) {
+        pin_lane_offset(jcb, r);
         bool gr;
         const int rs = mirror<0>(r, ng, ng + nx, gr);
         const long long ro = __builtin_amdgcn_readfirstlane(rs * s);
@@ -228,31 +230,36 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
     if (RECON == kReconMUSCL) S0 = slope_of(Pm1, P0, P1);
     FaceFlux Fm = xface(Pm1, RECON == kReconMUSCL ? slope_of(Pm2, Pm1, P0) : zero, P0, S0);
 
-#pragma unroll 1
-    for (int r = r0; r < r1; ++r) {
-        const MCell Pn = load(r + 3 <= rmax ? r + 3 : rmax);
+    // One output row: rows r (c0), r+1 (c1), r+2 (c2) are loaded, row r+3 goes into cn; s0 is
+    // the slope of row r, s1 receives that of row r+1; fm is face r-1/2, fx receives r+1/2.
+    auto row = [&](int r, const MCell &P0, const MCell &P1, const MCell &P2, MCell &Pn,
+                   const MSlope &S0, MSlope &S1, const FaceFlux &Fm, FaceFlux &Fx) {
+        Pn = load(r + 3 <= rmax ? r + 3 : rmax);
         if (RECON == kReconMUSCL) S1 = slope_of(P0, P1, P2);
-        const FaceFlux Fx = xface(P0, S0, P1, S1);
+        Fx = xface(P0, S0, P1, S1);
 
-        // y faces: lane j takes face j+1/2 (normal hv, tangential hu)
+        // y faces: lane j takes face j+1/2 (normal hv, tangential hu).  The lane moves fill
+        // the lane without a source with 0: lane 63's face, lane 62's (it reads lane 63's
+        // slope) and lane 0's slope and divergence belong to halo columns, whose results
+        // are never stored and whose flags are masked by `yface` / `owner`.
         NT m{P0.w, P0.hv, P0.hu};
-        double pw = from_next(P0.w), phv = from_next(P0.hv), phu = from_next(P0.hu);
+        double pw = from_next_z(P0.w), phv = from_next_z(P0.hv), phu = from_next_z(P0.hu);
         if (RECON == kReconMUSCL) {
             const double dw = pw - P0.w, dv = phv - P0.hv, du = phu - P0.hu;
-            const double sw = minmod(from_prev(dw), dw), sv = minmod(from_prev(dv), dv);
-            const double su = minmod(from_prev(du), du);
+            const double sw = minmod(from_prev_z(dw), dw), sv = minmod(from_prev_z(dv), dv);
+            const double su = minmod(from_prev_z(du), du);
             m = NT{P0.w + 0.5 * sw, P0.hv + 0.5 * sv, P0.hu + 0.5 * su};
-            pw = from_next(P0.w - 0.5 * sw);
-            phv = from_next(P0.hv - 0.5 * sv);
-            phu = from_next(P0.hu - 0.5 * su);
+            pw = from_next_z(P0.w - 0.5 * sw);
+            phv = from_next_z(P0.hv - 0.5 * sv);
+            phu = from_next_z(P0.hu - 0.5 * su);
         }
-        const double pb = BATHY ? from_next(P0.b) : 0.0;
+        const double pb = BATHY ? from_next_z(P0.b) : 0.0;
         fbad = 0;
         const FaceFlux Fy = face_mod<1, RIEMANN, BATHY>(g, half_g, m, P0.b, NT{pw, phv, phu}, pb,
                                                         fbad);
         bad |= yface ? fbad : 0;
-        const double fyp_h = from_prev(Fy.fh), fyp_hu = from_prev(Fy.ft);
-        const double fyp_hv = from_prev(Fy.fn_p);
+        const double fyp_h = from_prev_z(Fy.fh), fyp_hu = from_prev_z(Fy.ft);
+        const double fyp_hv = from_prev_z(Fy.fn_p);
 
         // finite_volume(_bathy).hpp
         const double rh = divergence(G.inv_dx, G.inv_dy, Fx.fh, Fm.fh, Fy.fh, fyp_h);
@@ -261,6 +268,7 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
 
         if (owner) {
             const long long ro = __builtin_amdgcn_readfirstlane(r * s);
+            pin_lane_offset(jb, r);
             if constexpr (STAGE == 0) {
                 st(row_ptr(A.out_h, ro), jb, rh);
                 st(row_ptr(A.out_hu, ro), jb, rhu);
@@ -293,16 +301,26 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
                     }
                     if (!isfinite(hN) || !isfinite(huN) || !isfinite(hvN)) bad = 1;
                 }
+                pin_lane_offset(jb, r);
                 st(row_ptr(A.out_h, ro), jb, hN);
                 st(row_ptr(A.out_hu, ro), jb, huN);
                 st(row_ptr(A.out_hv, ro), jb, hvN);
             }
         }
-        Fm = Fx;
-        P0 = P1;
-        P1 = P2;
-        P2 = Pn;
-        S0 = S1;
+    };
+
+    // march, unrolled 4x so the register window rotates by renaming instead of copies
+    MCell P3;
+    FaceFlux F1;
+    FaceFlux &F0 = Fm;
+    for (int r = r0; r < r1; r += 4) {
+        row(r, P0, P1, P2, P3, S0, S1, F0, F1);
+        if (r + 1 >= r1) break;
+        row(r + 1, P1, P2, P3, P0, S1, S0, F1, F0);
+        if (r + 2 >= r1) break;
+        row(r + 2, P2, P3, P0, P1, S0, S1, F0, F1);
+        if (r + 3 >= r1) break;
+        row(r + 3, P3, P0, P1, P2, S1, S0, F1, F0);
     }
     if (__any(bad)) {
         if (lane == 0) atomicOr(A.flag, 1);
