@@ -299,7 +299,8 @@ template <int BC>
 struct Lanes {
     const AdjArgs &A;
     int s, ng, nx, ny, lane, j0;
-    unsigned j_bytes, jc_bytes;  // own column / mirrored source column, in bytes
+    // own column / mirrored source column, in bytes; re-pinned per row (pin_lane_offset)
+    mutable unsigned j_bytes, jc_bytes;
     bool gcol;   // the lane's column is outside the interior (its values are mirrored)
     bool owner;  // the lane writes its column
     bool xact;   // interior column: x faces live, lamW nonzero
@@ -325,6 +326,7 @@ struct Lanes {
 
     // Row r of V (r within 2 of the interior); `floor` = the stored cell is the floor state.
     __device__ __forceinline__ PCell load(int r, bool &floor) const {
+        pin_lane_offset(jc_bytes, r);
         bool gr;
         const int rs = mirror<BC>(r, ng, ng + nx, gr);
         const long long ro = row_off(rs);
@@ -349,6 +351,7 @@ struct Lanes {
         if (r < ng || r >= ng + nx) return l;  // wave-uniform
         const long long ro = row_off(r);
         if (xact) {
+            pin_lane_offset(j_bytes, r);
             const bool m = A.mask_out[ro + (j_bytes >> 3)] != 0;
             const double a = ld(row_ptr(A.lam_h, ro), j_bytes);
             const double b = ld(row_ptr(A.lam_hu, ro), j_bytes);
@@ -360,8 +363,15 @@ struct Lanes {
         return l;
     }
 
-    __device__ __forceinline__ static bool split_row(const PCell &c) {
-        return split_ok(c.eta - c.b) && split_ok(c.hu) && split_ok(c.hv) && split_ok(c.b);
+    // The lanes whose cell is inside the split forms' range guard (depth h = eta - b,
+    // momenta, bottom), one ballot per compare.
+    __device__ __forceinline__ static unsigned long long split_lanes(const PCell &c) {
+        return split_ok_lanes(c.eta - c.b) & split_ok_lanes(c.hu) & split_ok_lanes(c.hv) &
+               split_ok_lanes(c.b);
+    }
+    __device__ __forceinline__ static bool all_lanes(unsigned long long m) {
+        const unsigned long long exec = __builtin_amdgcn_read_exec();
+        return (m & exec) == exec;
     }
 
     // y faces of one row: the adjoint contributions to the row's cells, ghost columns
@@ -369,37 +379,39 @@ struct Lanes {
     template <bool SPLIT>
     __device__ __forceinline__ A4 y_row(double half_g, double g, double ky, const PCell &P,
                                         const D3 &L) const {
-        const double de = from_next(P.eta) - P.eta;
-        const double du = from_next(P.hu) - P.hu;
-        const double dv = from_next(P.hv) - P.hv;
+        // The lane moves fill the lane without a source (63 resp. 0) with 0.  Those lanes are
+        // in the 3-column halo, like every lane their values reach, and no vote reads them.
+        const double de = from_next_z(P.eta) - P.eta;
+        const double du = from_next_z(P.hu) - P.hu;
+        const double dv = from_next_z(P.hv) - P.hv;
         int se_, su_, sv_;
-        const double se = half_minmod(from_prev(de), de, se_);
-        const double su = half_minmod(from_prev(du), du, su_);
-        const double sv = half_minmod(from_prev(dv), dv, sv_);
-        const double pe = from_next(P.eta - se);
-        const double pu = from_next(P.hu - su);
-        const double pv = from_next(P.hv - sv);
-        const double pb = from_next(P.b);
-        const double le = from_next(L.e), lu = from_next(L.u), lv = from_next(L.v);
+        const double se = half_minmod(from_prev_z(de), de, se_);
+        const double su = half_minmod(from_prev_z(du), du, su_);
+        const double sv = half_minmod(from_prev_z(dv), dv, sv_);
+        const double pe = from_next_z(P.eta - se);
+        const double pu = from_next_z(P.hu - su);
+        const double pv = from_next_z(P.hv - sv);
+        const double pb = from_next_z(P.b);
+        const double le = from_next_z(L.e), lu = from_next_z(L.u), lv = from_next_z(L.v);
         // y faces: normal momentum hv, tangential hu
         FaceAdj f = face_adjoint<SPLIT>(half_g, g, P.eta + se, P.hv + sv, P.hu + su, P.b, pe, pv,
                                         pu, pb, -ky * L.e, -ky * L.v, -ky * L.u, ky * le,
                                         ky * lv, ky * lu);
         if (!yact) f = face_zero();
         // plus-side parts of the face below (lane j-1) belong to this lane's cell
-        const double qe = from_prev(f.ep), qn = from_prev(f.np), qt = from_prev(f.tp);
-        const double qb = from_prev(f.bp);
+        const double qe = from_prev_z(f.ep), qn = from_prev_z(f.np), qt = from_prev_z(f.tp);
+        const double qb = from_prev_z(f.bp);
         // slope adjoints of this cell (the minus state uses +sigma, the plus state -sigma),
         // routed by the minmod selection: own part, lane j+1's "previous" part and lane
         // j-1's "next" part land here
         A4 acc;
         double tp, ts, tn;
         slope_parts(se_, f.em - qe, tp, ts, tn);
-        acc.e = f.em + qe + ts + from_next(tp) + from_prev(tn);
+        acc.e = f.em + qe + ts + from_next_z(tp) + from_prev_z(tn);
         slope_parts(su_, f.tm - qt, tp, ts, tn);
-        acc.u = f.tm + qt + ts + from_next(tp) + from_prev(tn);
+        acc.u = f.tm + qt + ts + from_next_z(tp) + from_prev_z(tn);
         slope_parts(sv_, f.nm - qn, tp, ts, tn);
-        acc.v = f.nm + qn + ts + from_next(tp) + from_prev(tn);
+        acc.v = f.nm + qn + ts + from_next_z(tp) + from_prev_z(tn);
         acc.b = f.bm + qb;
         fold_cols(acc);
         return acc;
@@ -539,7 +551,8 @@ __device__ __forceinline__ void emit_cell(const AdjArgs &A, const Lanes<BC> &w, 
                                           const D3 &L, const A4 &acc, const A4 &T) {
     constexpr double cv = STAGE == 1 ? 1.0 : (STAGE == 2 ? 0.25 : 2.0 / 3.0);
     const long long ro = w.row_off(r);
-    const unsigned jb = w.j_bytes;
+    unsigned jb = w.j_bytes;
+    pin_lane_offset(jb, r);
     double oh = cv * L.e + (T.e + acc.e);
     double ou = cv * L.u + (T.u + acc.u);
     double ov = cv * L.v + (T.v + acc.v);
@@ -589,7 +602,7 @@ void adjoint_y_kernel(const GridDev G, const AdjArgs A) {
         const int rn = r + 1 < r1 ? r + 1 : r;
         const PCell Pn = w.load(rn, fl);
         const D3 Ln = w.load_lam(rn);
-        A4 acc = (G.split_ok && wave_all(Lanes<BC>::split_row(P)))
+        A4 acc = (G.split_ok && Lanes<BC>::all_lanes(Lanes<BC>::split_lanes(P)))
                      ? w.template y_row<true>(G.half_g, G.g, ky, P, L)
                      : w.template y_row<false>(G.half_g, G.g, ky, P, L);
         if (w.owner) emit_cell<STAGE, BC>(A, w, r, L, acc, gather_folded<BC>(X, G, r));
@@ -630,15 +643,16 @@ void adjoint_xface_kernel(const GridDev G, const AdjArgs A) {
     int sel0, sel1;
     {
         const PCell Pm = w.load(f0 - 1, fl);
-        split = split && wave_all(Lanes<BC>::split_row(Pm));
+        split = split && Lanes<BC>::all_lanes(Lanes<BC>::split_lanes(Pm));
         P0 = w.load(f0, fl);
         mark(f0, fl);
         P1 = w.load(f0 + 1, fl);
         mark(f0 + 1, fl);
         P2 = w.load(f0 + 2, fl);
         mark(f0 + 2, fl);
-        split = split && wave_all(Lanes<BC>::split_row(P0) && Lanes<BC>::split_row(P1) &&
-                               Lanes<BC>::split_row(P2));
+        split = split && Lanes<BC>::all_lanes(Lanes<BC>::split_lanes(P0) &
+                                                   Lanes<BC>::split_lanes(P1) &
+                                                   Lanes<BC>::split_lanes(P2));
         int sa, sb, sc;
         S0.e = half_minmod(P0.eta - Pm.eta, P1.eta - P0.eta, sa);
         S0.u = half_minmod(P0.hu - Pm.hu, P1.hu - P0.hu, sb);
@@ -688,7 +702,7 @@ void adjoint_xface_kernel(const GridDev G, const AdjArgs A) {
             sr[w.j_bytes >> 3] = (unsigned char)sel0;
             if (f == ng + nx - 1) sr[w.s + (w.j_bytes >> 3)] = (unsigned char)sel1;
         }
-        split = split && wave_all(Lanes<BC>::split_row(P3));
+        split = split && Lanes<BC>::all_lanes(Lanes<BC>::split_lanes(P3));
         P0 = P1;
         P1 = P2;
         P2 = P3;
@@ -741,8 +755,9 @@ void adjoint_stage_kernel(const GridDev G, const AdjArgs A) {
                 P0 = w.load(fa, fl);
                 P1 = w.load(fa + 1, fl);
                 P2 = w.load(fa + 2, fl);
-                split = split && wave_all(Lanes<BC>::split_row(Pm) && Lanes<BC>::split_row(P0) &&
-                                       Lanes<BC>::split_row(P1) && Lanes<BC>::split_row(P2));
+                split = split && Lanes<BC>::all_lanes(
+                                    Lanes<BC>::split_lanes(Pm) & Lanes<BC>::split_lanes(P0) &
+                                    Lanes<BC>::split_lanes(P1) & Lanes<BC>::split_lanes(P2));
                 int sa, sb, sc;
                 S0.e = half_minmod(P0.eta - Pm.eta, P1.eta - P0.eta, sa);
                 S0.u = half_minmod(P0.hu - Pm.hu, P1.hu - P0.hu, sb);
@@ -783,7 +798,7 @@ void adjoint_stage_kernel(const GridDev G, const AdjArgs A) {
                 slot[7 * kWave] = x.bp;
                 lds_sel[(f - F0) * kWave + lane] = (unsigned char)sel0;
                 if (f == F1) lds_sel[(f + 1 - F0) * kWave + lane] = (unsigned char)sel1;
-                split = split && wave_all(Lanes<BC>::split_row(P3));
+                split = split && Lanes<BC>::all_lanes(Lanes<BC>::split_lanes(P3));
                 P0 = P1;
                 P1 = P2;
                 P2 = P3;
@@ -810,7 +825,7 @@ void adjoint_stage_kernel(const GridDev G, const AdjArgs A) {
         const int rn = r + 1 < rb ? r + 1 : r;
         const PCell Pn = w.load(rn, fln);
         const D3 Ln = w.load_lam(rn);
-        A4 acc = (G.split_ok && wave_all(Lanes<BC>::split_row(P)))
+        A4 acc = (G.split_ok && Lanes<BC>::all_lanes(Lanes<BC>::split_lanes(P)))
                      ? w.template y_row<true>(G.half_g, G.g, ky, P, L)
                      : w.template y_row<false>(G.half_g, G.g, ky, P, L);
         if (w.owner) {
