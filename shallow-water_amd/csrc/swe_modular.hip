@@ -20,9 +20,12 @@
 //     1/3 U + 2/3 U2 + (2 dt/3) L;
 //   * the positivity floor and non-finite check only with bathymetry (serial_solver.hpp:
 //     190-206); the plain path has neither.
-// Division and sqrt are the compiler's IEEE sequences (no split forms here).  Where the
-// reference's HLL throws (NaN wave speeds, HLL.hpp:23), the stage raises the non-finite
-// flag instead.
+// Division and sqrt: the split forms of swe_face.h (same bits as `/` and sqrt()) for waves
+// whose loaded cells all pass the modular range guard (mod_split_lanes below), the
+// compiler's IEEE sequences otherwise.  Where the reference's HLL throws (NaN wave speeds,
+// HLL.hpp:23), the stage raises the non-finite flag instead.
+#include <type_traits>
+
 #include "swe_face.h"
 #include "swe_internal.h"
 #include "swe_lanes.h"
@@ -38,18 +41,64 @@ struct NT {
     double h, n, t;
 };
 
-// CellState::u / v (cell_state.hpp:19-26)
-__device__ __forceinline__ double cs_vel(double q, double h) { return (h > kEps) ? (q / h) : 0.0; }
-// flux_assembly_bathy.hpp:233-240
-__device__ __forceinline__ double fa_vel(double q, double h) { return (h < kEps) ? 0.0 : q / h; }
+// ---- division and square root -----------------------------------------------------------
+// SPLIT: the reduced sequences of swe_face.h (rcp + 4 fma per denominator, shared by the
+// divisions with that denominator; mul, fma, fma per numerator; sqrt without its rescale).
+// They equal `/` and sqrt() bit for bit when d > 0, n/d and n are normal or zero, the
+// exponents are far from the limits and sqrt's argument is 0 or >= 2^-767.  The modular
+// range guard makes that hold for every division and root these schemes form at a wave
+// whose loaded depths are in [2^-100, 2^100] and momenta and bottoms 0 or in that range:
+//  * reconstructed values are multiples of 2^-153 of magnitude <= 2^102; minmod-limited
+//    MUSCL keeps a reconstructed depth between its neighbours, so plain depths stay
+//    >= 2^-153; hydrostatic depths max(eta - b, 0) are 0 or >= 2^-153;
+//  * velocities q/h are taken only for h > eps = 1e-8 (the selects discard the others):
+//    |u| <= 2^102 / 2^-26.5 < 2^129, nonzero |u| >= 2^-255;
+//  * sqrt(g h) with g in [2^-10, 2^10] (checked on the host): 0 or >= 2^-82;
+//  * HLL's star divides by d = sR - sL > 0 where sL < 0 < sR, and d >= 2 max(cL, cR), so
+//    d >= 2^-81 unless both sides are dry (then sL = sR = 0 and no star); every flux
+//    component is < 2^234 and every quotient (N/d, |N/d| <= |FL| + |FR| + |sL dU|) < 2^236,
+//    while a nonzero N is a rounded sum of products of at least 2^-82 * 2^-255, far above
+//    2^-969 with a quotient above 2^-600;
+//  * ROE divides by sqrt(h1) + sqrt(h2) >= 2^-77 and by c = sqrt(g h) >= 2^-81, with
+//    numerators below 2^240.
+template <bool SPLIT>
+__device__ __forceinline__ double mrecip(double d) {
+    if constexpr (SPLIT) return div_denominator(d);
+    return 0.0;  // unused: the IEEE division needs no separate reciprocal
+}
+template <bool SPLIT>
+__device__ __forceinline__ double mdiv(double n, double d, double r) {
+    if constexpr (SPLIT) return div_numerator(n, d, r);
+    return n / d;
+}
+
+// The lanes whose cell passes the modular range guard (above), one ballot per compare.
+__device__ __forceinline__ unsigned long long mod_in_range(double v) {
+    const double a = fabs(v);
+    return __builtin_amdgcn_ballot_w64(a >= 0x1p-100) & __builtin_amdgcn_ballot_w64(a <= 0x1p100);
+}
+__device__ __forceinline__ unsigned long long mod_zero_or_in_range(double v) {
+    return mod_in_range(v) | __builtin_amdgcn_ballot_w64(v == 0.0);
+}
+
+// CellState::u / v (cell_state.hpp:19-26); r = mrecip<SPLIT>(h)
+template <bool SPLIT>
+__device__ __forceinline__ double cs_vel(double q, double h, double r) {
+    return (h > kEps) ? mdiv<SPLIT>(q, h, r) : 0.0;
+}
+// flux_assembly_bathy.hpp:233-240; r = mrecip<SPLIT>(h)
+template <bool SPLIT>
+__device__ __forceinline__ double fa_vel(double q, double h, double r) {
+    return (h < kEps) ? 0.0 : mdiv<SPLIT>(q, h, r);
+}
 // flux_assembly_bathy.hpp:231 std::max(0.0, h)
 __device__ __forceinline__ double fa_positive(double h) { return (0.0 < h) ? h : 0.0; }
 
 // riemann.hpp:12-19 in normal/tangential form: x uses F = (hu, u*u*h + g/2 h^2, hu*v);
 // y uses G = (hv, hu*v, h*v*v + g/2 h^2), i.e. (n, t*un, h*un*un + ...).
-template <int DIR>
-__device__ __forceinline__ NT phys(double half_g, const NT &U) {
-    const double un = cs_vel(U.n, U.h), ut = cs_vel(U.t, U.h);
+template <int DIR, bool SPLIT>
+__device__ __forceinline__ NT phys(double half_g, const NT &U, double r) {
+    const double un = cs_vel<SPLIT>(U.n, U.h, r), ut = cs_vel<SPLIT>(U.t, U.h, r);
     NT f;
     f.h = U.n;
     if (DIR == 0) {
@@ -63,19 +112,22 @@ __device__ __forceinline__ NT phys(double half_g, const NT &U) {
 }
 
 // HLL.hpp:9-45
-template <int DIR>
+template <int DIR, bool SPLIT>
 __device__ __forceinline__ NT hll_mod(double g, double half_g, const NT &L, const NT &R,
                                       int &bad) {
-    const double uL = cs_vel(L.n, L.h), uR = cs_vel(R.n, R.h);
-    const double sR = smax(uL + sqrt(g * L.h), uR + sqrt(g * R.h));
-    const double sL = smin(uL - sqrt(g * L.h), uR - sqrt(g * R.h));
-    const NT FL = phys<DIR>(half_g, L), FR = phys<DIR>(half_g, R);
+    const double rL = mrecip<SPLIT>(L.h), rR = mrecip<SPLIT>(R.h);
+    const double uL = cs_vel<SPLIT>(L.n, L.h, rL), uR = cs_vel<SPLIT>(R.n, R.h, rR);
+    const double cL = ieee_sqrt<SPLIT>(g * L.h), cR = ieee_sqrt<SPLIT>(g * R.h);
+    const double sR = smax(uL + cL, uR + cR);
+    const double sL = smin(uL - cL, uR - cR);
+    const NT FL = phys<DIR, SPLIT>(half_g, L, rL), FR = phys<DIR, SPLIT>(half_g, R, rR);
     if (sL >= 0.0) return FL;
     if (sL < 0.0 && sR > 0.0) {
         const double d = sR - sL, ss = sL * sR;
-        return NT{(sR * FL.h - sL * FR.h + ss * (R.h - L.h)) / d,
-                  (sR * FL.n - sL * FR.n + ss * (R.n - L.n)) / d,
-                  (sR * FL.t - sL * FR.t + ss * (R.t - L.t)) / d};
+        const double rd = mrecip<SPLIT>(d);
+        return NT{mdiv<SPLIT>(sR * FL.h - sL * FR.h + ss * (R.h - L.h), d, rd),
+                  mdiv<SPLIT>(sR * FL.n - sL * FR.n + ss * (R.n - L.n), d, rd),
+                  mdiv<SPLIT>(sR * FL.t - sL * FR.t + ss * (R.t - L.t), d, rd)};
     }
     if (sR <= 0.0) return FR;
     bad = 1;  // the reference throws "HLL could not provide dissipation speed."
@@ -83,61 +135,69 @@ __device__ __forceinline__ NT hll_mod(double g, double half_g, const NT &L, cons
 }
 
 // rusanov.hpp:11-30
-template <int DIR>
+template <int DIR, bool SPLIT>
 __device__ __forceinline__ NT rusanov(double g, double half_g, const NT &L, const NT &R) {
-    const double uL = cs_vel(L.n, L.h), uR = cs_vel(R.n, R.h);
+    const double rL = mrecip<SPLIT>(L.h), rR = mrecip<SPLIT>(R.h);
+    const double uL = cs_vel<SPLIT>(L.n, L.h, rL), uR = cs_vel<SPLIT>(R.n, R.h, rR);
     const double hL = (L.h < 0.0) ? 0.0 : L.h, hR = (R.h < 0.0) ? 0.0 : R.h;  // std::max(h, 0)
-    const double a = smax(fabs(uL) + sqrt(g * hL), fabs(uR) + sqrt(g * hR));
-    const NT FL = phys<DIR>(half_g, L), FR = phys<DIR>(half_g, R);
+    const double a = smax(fabs(uL) + ieee_sqrt<SPLIT>(g * hL), fabs(uR) + ieee_sqrt<SPLIT>(g * hR));
+    const NT FL = phys<DIR, SPLIT>(half_g, L, rL), FR = phys<DIR, SPLIT>(half_g, R, rR);
     const double ha = 0.5 * a;
     return NT{0.5 * (FL.h + FR.h) - ha * (R.h - L.h), 0.5 * (FL.n + FR.n) - ha * (R.n - L.n),
               0.5 * (FL.t + FR.t) - ha * (R.t - L.t)};
 }
 
-// ROE.hpp:66-69
-__device__ __forceinline__ double roe_avg(double u1, double h1, double u2, double h2) {
-    return ((sqrt(h1) * u1) + (sqrt(h2) * u2)) / (sqrt(h1) + sqrt(h2));
+// ROE.hpp:66-69, given q1 = sqrt(h1), q2 = sqrt(h2), r = mrecip<SPLIT>(q1 + q2)
+template <bool SPLIT>
+__device__ __forceinline__ double roe_avg(double u1, double q1, double u2, double q2, double r) {
+    return mdiv<SPLIT>((q1 * u1) + (q2 * u2), q1 + q2, r);
 }
 
 // ROE.hpp:9-61.  In normal/tangential form both directions read the same: eigenvectors
 // r1 = (1, l1, ut), r2 = (0, 0, 1), r3 = (1, l3, ut); the reference's array operators
 // multiply element * scalar and add commutatively, so these sums are its sums.
-template <int DIR>
+template <int DIR, bool SPLIT>
 __device__ __forceinline__ NT roe(double g, double half_g, const NT &L, const NT &R) {
     const double h = (L.h + R.h) / 2.0;
-    const double c = sqrt(g * h);
-    const double un = roe_avg(cs_vel(L.n, L.h), L.h, cs_vel(R.n, R.h), R.h);
-    const double ut = roe_avg(cs_vel(L.t, L.h), L.h, cs_vel(R.t, R.h), R.h);
+    const double c = ieee_sqrt<SPLIT>(g * h);
+    const double rL = mrecip<SPLIT>(L.h), rR = mrecip<SPLIT>(R.h);
+    const double qL = ieee_sqrt<SPLIT>(L.h), qR = ieee_sqrt<SPLIT>(R.h);
+    const double rq = mrecip<SPLIT>(qL + qR);
+    const double un = roe_avg<SPLIT>(cs_vel<SPLIT>(L.n, L.h, rL), qL, cs_vel<SPLIT>(R.n, R.h, rR),
+                                     qR, rq);
+    const double ut = roe_avg<SPLIT>(cs_vel<SPLIT>(L.t, L.h, rL), qL, cs_vel<SPLIT>(R.t, R.h, rR),
+                                     qR, rq);
     const double l1 = un - c, l2 = un, l3 = un + c;
     const double dh = R.h - L.h, dn = R.n - L.n, dt = R.t - L.t;
-    const double a1 = 0.5 * (dh - ((dn - un * dh) / c));
-    const double a3 = 0.5 * (dh + ((dn - un * dh) / c));
+    const double w = mdiv<SPLIT>(dn - un * dh, c, mrecip<SPLIT>(c));
+    const double a1 = 0.5 * (dh - w);
+    const double a3 = 0.5 * (dh + w);
     const double a2 = dt - ut * dh;
     const double s1 = fabs(l1) * a1, s2 = fabs(l2) * a2, s3 = fabs(l3) * a3;
     const double sh = ((1.0 * s1) + (0.0 * s2)) + (1.0 * s3);
     const double sn = ((l1 * s1) + (0.0 * s2)) + (l3 * s3);
     const double st_ = ((ut * s1) + (1.0 * s2)) + (ut * s3);
-    const NT FL = phys<DIR>(half_g, L), FR = phys<DIR>(half_g, R);
+    const NT FL = phys<DIR, SPLIT>(half_g, L, rL), FR = phys<DIR, SPLIT>(half_g, R, rR);
     return NT{0.5 * (FL.h + FR.h) - sh * 0.5, 0.5 * (FL.n + FR.n) - sn * 0.5,
               0.5 * (FL.t + FR.t) - st_ * 0.5};
 }
 
-template <int DIR, int RIEMANN>
+template <int DIR, int RIEMANN, bool SPLIT>
 __device__ __forceinline__ NT riemann(double g, double half_g, const NT &L, const NT &R,
                                       int &bad) {
-    if constexpr (RIEMANN == kRusanov) return rusanov<DIR>(g, half_g, L, R);
-    else if constexpr (RIEMANN == kROE) return roe<DIR>(g, half_g, L, R);
-    else return hll_mod<DIR>(g, half_g, L, R, bad);
+    if constexpr (RIEMANN == kRusanov) return rusanov<DIR, SPLIT>(g, half_g, L, R);
+    else if constexpr (RIEMANN == kROE) return roe<DIR, SPLIT>(g, half_g, L, R);
+    else return hll_mod<DIR, SPLIT>(g, half_g, L, R, bad);
 }
 
 // One face from the reconstructed minus/plus states (w = h, or eta with bathymetry) and
 // the cell bottoms.  Returns (fh, ft, fn_minus, fn_plus) like the optimized face.
-template <int DIR, int RIEMANN, bool BATHY>
+template <int DIR, int RIEMANN, bool BATHY, bool SPLIT>
 __device__ __forceinline__ FaceFlux face_mod(double g, double half_g, const NT &m, double b_m,
                                              const NT &p, double b_p, int &bad) {
     FaceFlux f;
     if constexpr (!BATHY) {
-        const NT F = riemann<DIR, RIEMANN>(g, half_g, m, p, bad);  // flux_assembly.hpp:19-25
+        const NT F = riemann<DIR, RIEMANN, SPLIT>(g, half_g, m, p, bad);  // flux_assembly.hpp:19-25
         f.fh = F.h;
         f.ft = F.t;
         f.fn_m = F.n;
@@ -146,11 +206,12 @@ __device__ __forceinline__ FaceFlux face_mod(double g, double half_g, const NT &
         // flux_assembly_bathy.hpp:35-88
         const double b_star = smax(b_m, b_p);
         const double h_m = fa_positive(m.h - b_m), h_p = fa_positive(p.h - b_p);
-        const double un_m = fa_vel(m.n, h_m), ut_m = fa_vel(m.t, h_m);
-        const double un_p = fa_vel(p.n, h_p), ut_p = fa_vel(p.t, h_p);
+        const double r_m = mrecip<SPLIT>(h_m), r_p = mrecip<SPLIT>(h_p);
+        const double un_m = fa_vel<SPLIT>(m.n, h_m, r_m), ut_m = fa_vel<SPLIT>(m.t, h_m, r_m);
+        const double un_p = fa_vel<SPLIT>(p.n, h_p, r_p), ut_p = fa_vel<SPLIT>(p.t, h_p, r_p);
         const double hs_m = fa_positive(m.h - b_star), hs_p = fa_positive(p.h - b_star);
         const NT Um{hs_m, hs_m * un_m, hs_m * ut_m}, Up{hs_p, hs_p * un_p, hs_p * ut_p};
-        const NT F = riemann<DIR, RIEMANN>(g, half_g, Um, Up, bad);
+        const NT F = riemann<DIR, RIEMANN, SPLIT>(g, half_g, Um, Up, bad);
         f.fh = F.h;
         f.ft = F.t;
         f.fn_m = F.n + half_g * (h_m * h_m - hs_m * hs_m);
@@ -193,6 +254,12 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
     const double dt = STAGE != 0 ? *A.dt : 0.0;
     int bad = 0;   // HLL without dissipation speed, or a non-finite stage output
     int fbad = 0;  // per face, then masked
+    // Wave-uniform, sticky: every cell loaded so far by this wave passes the modular range
+    // guard, so its faces may use the split division and sqrt forms.  Rusanov has too few
+    // divisions for the split forms to pay for the guard (measured 4% slower with
+    // piecewise-constant reconstruction): it keeps the IEEE sequences.
+    constexpr bool kSplitScheme = RIEMANN != kRusanov;
+    bool split = kSplitScheme && G.split_ok != 0;
 
     auto load = [&](int r) {
         pin_lane_offset(jcb, r);
@@ -207,17 +274,26 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
         if (gcol) c.hv = -c.hv;
         c.b = BATHY ? ld(row_ptr(A.B, ro), jcb) : 0.0;
         c.w = BATHY ? h + c.b : h;
+        if constexpr (kSplitScheme) {
+            unsigned long long ok = mod_in_range(h) & mod_zero_or_in_range(c.hu) &
+                                    mod_zero_or_in_range(c.hv);
+            if (BATHY) ok &= mod_zero_or_in_range(c.b);
+            const unsigned long long exec = __builtin_amdgcn_read_exec();
+            split = split && (ok & exec) == exec;
+        }
         return c;
     };
     // x face between rows a (minus, slope sa) and b (plus, slope sb): normal hu
-    auto xface = [&](const MCell &a, const MSlope &sa, const MCell &b, const MSlope &sb) {
+    auto xface = [&](auto split_tag, const MCell &a, const MSlope &sa, const MCell &b,
+                     const MSlope &sb) {
+        constexpr bool SPLIT = decltype(split_tag)::value;
         NT m{a.w, a.hu, a.hv}, p{b.w, b.hu, b.hv};
         if (RECON == kReconMUSCL) {
             m = NT{a.w + 0.5 * sa.w, a.hu + 0.5 * sa.hu, a.hv + 0.5 * sa.hv};
             p = NT{b.w - 0.5 * sb.w, b.hu - 0.5 * sb.hu, b.hv - 0.5 * sb.hv};
         }
         fbad = 0;
-        const FaceFlux f = face_mod<0, RIEMANN, BATHY>(g, half_g, m, a.b, p, b.b, fbad);
+        const FaceFlux f = face_mod<0, RIEMANN, BATHY, SPLIT>(g, half_g, m, a.b, p, b.b, fbad);
         bad |= owner ? fbad : 0;
         return f;
     };
@@ -228,15 +304,23 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
     MCell P0 = load(r0), P1 = load(r0 + 1), P2 = load(r0 + 2 <= rmax ? r0 + 2 : rmax);
     MSlope S0 = zero, S1 = zero;
     if (RECON == kReconMUSCL) S0 = slope_of(Pm1, P0, P1);
-    FaceFlux Fm = xface(Pm1, RECON == kReconMUSCL ? slope_of(Pm2, Pm1, P0) : zero, P0, S0);
+    const MSlope Sm1 = RECON == kReconMUSCL ? slope_of(Pm2, Pm1, P0) : zero;
+    FaceFlux Fm;
+    if constexpr (kSplitScheme) {
+        Fm = split ? xface(std::true_type{}, Pm1, Sm1, P0, S0)
+                   : xface(std::false_type{}, Pm1, Sm1, P0, S0);
+    } else {
+        Fm = xface(std::false_type{}, Pm1, Sm1, P0, S0);
+    }
 
     // One output row: rows r (c0), r+1 (c1), r+2 (c2) are loaded, row r+3 goes into cn; s0 is
     // the slope of row r, s1 receives that of row r+1; fm is face r-1/2, fx receives r+1/2.
-    auto row = [&](int r, const MCell &P0, const MCell &P1, const MCell &P2, MCell &Pn,
-                   const MSlope &S0, MSlope &S1, const FaceFlux &Fm, FaceFlux &Fx) {
-        Pn = load(r + 3 <= rmax ? r + 3 : rmax);
+    auto row_body = [&](auto split_tag, int r, const MCell &P0, const MCell &P1,
+                        const MCell &P2, const MSlope &S0, MSlope &S1, const FaceFlux &Fm,
+                        FaceFlux &Fx) {
+        constexpr bool SPLIT = decltype(split_tag)::value;
         if (RECON == kReconMUSCL) S1 = slope_of(P0, P1, P2);
-        Fx = xface(P0, S0, P1, S1);
+        Fx = xface(split_tag, P0, S0, P1, S1);
 
         // y faces: lane j takes face j+1/2 (normal hv, tangential hu).  The lane moves fill
         // the lane without a source with 0: lane 63's face, lane 62's (it reads lane 63's
@@ -255,8 +339,8 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
         }
         const double pb = BATHY ? from_next_z(P0.b) : 0.0;
         fbad = 0;
-        const FaceFlux Fy = face_mod<1, RIEMANN, BATHY>(g, half_g, m, P0.b, NT{pw, phv, phu}, pb,
-                                                        fbad);
+        const FaceFlux Fy = face_mod<1, RIEMANN, BATHY, SPLIT>(g, half_g, m, P0.b,
+                                                               NT{pw, phv, phu}, pb, fbad);
         bad |= yface ? fbad : 0;
         const double fyp_h = from_prev_z(Fy.fh), fyp_hu = from_prev_z(Fy.ft);
         const double fyp_hv = from_prev_z(Fy.fn_p);
@@ -306,6 +390,18 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
                 st(row_ptr(A.out_hu, ro), jb, huN);
                 st(row_ptr(A.out_hv, ro), jb, hvN);
             }
+        }
+    };
+    auto row = [&](int r, const MCell &P0, const MCell &P1, const MCell &P2, MCell &Pn,
+                   const MSlope &S0, MSlope &S1, const FaceFlux &Fm, FaceFlux &Fx) {
+        Pn = load(r + 3 <= rmax ? r + 3 : rmax);
+        if constexpr (kSplitScheme) {
+            if (split)
+                row_body(std::true_type{}, r, P0, P1, P2, S0, S1, Fm, Fx);
+            else
+                row_body(std::false_type{}, r, P0, P1, P2, S0, S1, Fm, Fx);
+        } else {
+            row_body(std::false_type{}, r, P0, P1, P2, S0, S1, Fm, Fx);
         }
     };
 

@@ -72,6 +72,45 @@ def test_rhs_and_steps_match_oracle(hip, bathy, recon, riemann):
                 assert same(g_, w_), (nx, ny, tile, "steps", nm)
 
 
+@pytest.mark.parametrize("case", ["dry_cells", "tiny_momentum", "huge_momentum"])
+@pytest.mark.parametrize("bathy,recon,riemann", SUPPORTED)
+def test_range_guard_fallback(hip, bathy, recon, riemann, case):
+    """Waves that load a cell outside the modular range guard (a dry cell, a momentum below
+    2^-100 or above 2^100) take the full IEEE division and sqrt sequences, the others the
+    split forms; the results stay bitwise equal to the oracle either way."""
+    nx, ny = 40, 130
+    gr, h, hu, hv, B = wet_state(nx, ny, 91 + 3 * recon + riemann, bathy, 2.5, 8.125)
+    if case == "dry_cells":
+        h[9, 30] = h[20, 95] = h[31, 7] = 0.0   # isolated: ROE never divides 0 by 0
+        hu[9, 30] = hv[20, 95] = 0.0
+    elif case == "tiny_momentum":
+        hu[12, 40:50] = 3e-40
+        hv[25, 100:104] = -1e-35
+    else:
+        hv[15, 60:63] = 4e31
+    oracle.mod_apply_bc(gr, h, hu, hv)
+    with solver(hip, gr, bathy, recon, riemann) as s:
+        s.upload(h, hu, hv, B)
+        got = s.rhs()
+        rc, want = oracle.mod_rhs(gr, bathy, recon, riemann, h.copy(), hu.copy(), hv.copy(), B)
+        assert rc == 0
+        for g_, w_, nm in zip(got, want, ("h", "hu", "hv")):
+            assert same(g_, w_), (case, "rhs", nm)
+        if case == "huge_momentum":
+            return  # fluxes of ~1e63 leave no stable step to compare
+        dt = 0.02 * min(gr.dx, gr.dy)
+        s.upload(h, hu, hv, B)
+        ref = [a.copy() for a in (h, hu, hv)]
+        rc = oracle.mod_run(gr, bathy, recon, riemann, *ref, B, 2, dt)
+        if rc != 0:
+            with pytest.raises(hip.SweError):
+                s.step(dt, 2)
+            return
+        s.step(dt, 2)
+        for g_, w_, nm in zip(s.download(), ref, ("h", "hu", "hv")):
+            assert same(g_, w_), (case, "steps", nm)
+
+
 @pytest.mark.parametrize("bathy,recon,riemann", SUPPORTED)
 def test_reference_fixtures(hip, bathy, recon, riemann):
     """Directly against outputs of the reference's own modular code (wet case)."""
