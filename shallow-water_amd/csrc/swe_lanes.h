@@ -1,5 +1,5 @@
-// swe_lanes.h -- cross-lane moves, ghost-index mirroring and row-based addressing shared by
-// the stage kernel and its adjoint.
+// swe_lanes.h -- cross-lane moves, wave votes, ghost-index mirroring and row-based
+// addressing shared by the stage, modular-scheme and adjoint kernels.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -8,23 +8,11 @@ namespace swe {
 
 // Lane l receives lane l+1's value / lane l-1's value, through the DPP wavefront shifts
 // wave_shl:1 (0x130) and wave_shr:1 (0x138) of the GFX9 ISA: a VALU move, no LDS round
-// trip.  The lane without a source (63 resp. 0) keeps its own value; callers only consume
-// the results of lanes that have a source.
-template <int CTRL>
-__device__ __forceinline__ double dpp_move(double x) {
-    const int lo = __builtin_amdgcn_update_dpp(__double2loint(x), __double2loint(x), CTRL, 0xF,
-                                               0xF, false);
-    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(x), __double2hiint(x), CTRL, 0xF,
-                                               0xF, false);
-    return __hiloint2double(hi, lo);
-}
-__device__ __forceinline__ double from_next(double x) { return dpp_move<0x130>(x); }
-__device__ __forceinline__ double from_prev(double x) { return dpp_move<0x138>(x); }
-
-// The same moves with bound_ctrl set: the lane without a source receives 0.  One
-// v_mov_b32_dpp per half (the keep-own-value form above first copies the source into the
-// destination, two more moves per double).  Callers must not let lane 63 (from_next_z) or
-// lane 0 (from_prev_z) reach a result, a store or a wave-wide vote (see wave_all's skip).
+// trip.  bound_ctrl is set, so the lane without a source (63 resp. 0) receives 0: one
+// v_mov_b32_dpp per half (a form that keeps the destination lane's own value would first
+// copy the source into the destination, two more moves per double).  Callers must not let
+// lane 63 (from_next_z) or lane 0 (from_prev_z) reach a result, a store or a wave-wide
+// vote (see wave_all's skip).
 template <int CTRL>
 __device__ __forceinline__ double dpp_move_z(double x) {
     const int lo = __builtin_amdgcn_mov_dpp(__double2loint(x), CTRL, 0xF, 0xF, true);
