@@ -282,6 +282,44 @@ def test_lake_at_rest_wet_roundoff(hip):
 
 
 @pytest.mark.slow
+@pytest.mark.parametrize("bump", [False, True])
+def test_lake_4096_full_size_properties(hip, bump):
+    """Config 2 at its real size: 4096^2 on [0,10]^2, 32 seeded Gaussian bumps, transmissive
+    walls, CFL 0.4, 2000 steps -- fully wet (eta 2.0; at eta 0.5 the reference scheme itself
+    diverges, test_wetdry_divergence_matches_reference).  Too large for the oracle, so
+    size-independent properties: dt equals the oracle's CFL expression bitwise; without the
+    bump the lake stays at rest to round-off; with the 0.05 bump the state stays finite and
+    positive, the wave stays below its initial amplitude and does not reach the walls in
+    2000 steps (0.44 s), so mass is conserved to round-off in both runs."""
+    n = 4096
+    c = W.lake_wet_dry(n, seed=2, bump=bump, eta0=2.0)
+    gr = oracle_grid(c)
+    B = c["B"].copy()
+    oracle.apply_bc(gr, B)
+    dt_o, _ = oracle.stable_dt(gr, c["h"].copy(), c["hu"].copy(), c["hv"].copy(), 0.4)
+    I = (slice(2, n + 2), slice(2, n + 2))
+    mass0 = c["h"][I].sum()
+    with gpu_solver(hip, c) as s:
+        dt, _ = s.stable_dt(0.4)
+        assert dt == dt_o
+        s.step(dt, 2000)
+        h, hu, hv = s.download()
+    del c
+    assert np.isfinite(h).all() and np.isfinite(hu).all() and np.isfinite(hv).all()
+    eta = h[I] + B[I]
+    if not bump:
+        assert np.abs(eta - 2.0).max() < 1e-11
+        assert np.abs(hu[I]).max() < 1e-11 and np.abs(hv[I]).max() < 1e-11
+    else:
+        assert h[I].min() > 0.3
+        assert np.abs(eta - 2.0).max() < 0.05
+    rim = np.ones((n, n), bool)
+    rim[8:-8, 8:-8] = False
+    assert np.abs(eta[rim] - 2.0).max() < 1e-11   # nothing has reached the walls
+    assert abs(h[I].sum() - mass0) / mass0 < 1e-12
+
+
+@pytest.mark.slow
 def test_dam_break_8192_step_subset(hip):
     """Config 3 at its real size (8192^2), 2 steps, bitwise vs the oracle."""
     c = W.dam_break(8192)
