@@ -320,6 +320,24 @@ def test_lake_4096_full_size_properties(hip, bump):
 
 
 @pytest.mark.slow
+def test_dam_break_8192_full_run_properties(hip):
+    """Config 3 at its real size for all its 1000 steps (CFL 0.45 dt, reflecting walls):
+    the state stays finite, the depth stays between the two initial levels up to the wave
+    overshoot (no floor), and the walls conserve mass to round-off."""
+    c = W.dam_break(8192)
+    I = (slice(2, 8194), slice(2, 8194))
+    mass0 = c["h"][I].sum()
+    with gpu_solver(hip, c) as s:
+        dt, _ = s.stable_dt(0.45)
+        s.step(dt, 1000)
+        h, hu, hv = s.download()
+    del c
+    assert np.isfinite(h).all() and np.isfinite(hu).all() and np.isfinite(hv).all()
+    assert 0.9 < h[I].min() and h[I].max() < 2.1
+    assert abs(h[I].sum() - mass0) / mass0 < 1e-12
+
+
+@pytest.mark.slow
 def test_dam_break_8192_step_subset(hip):
     """Config 3 at its real size (8192^2), 2 steps, bitwise vs the oracle."""
     c = W.dam_break(8192)
