@@ -138,3 +138,27 @@ def test_batch_rejects_mixed_parameters(hip):
     with pytest.raises(hip.SweError) as e:
         hip.HipBatch([a, c])
     assert e.value.code == hip.SWE_ERR_INVALID
+
+
+@pytest.mark.slow
+def test_ensemble_512_full_size_bitwise(hip):
+    """Config 4 at its real size: 512 instances (sides log-uniform in [64, 1024], seed 4,
+    96.6 M cells), 500 steps with per-instance adaptive dt in one batch (autotuned launch
+    shape) against each instance advanced alone: every instance bitwise equal, and the
+    per-instance dt sequences with it."""
+    sides = W.ensemble_sides(512)
+    cases = [W.ensemble_member(i, int(n)) for i, n in enumerate(sides)]
+    b = make_batch(hip, cases)
+    try:
+        b.autotune()   # times stages 1-2 only: the state is untouched
+        hist = b.step_adaptive(0.4, 500)
+        for i, c in enumerate(cases):
+            with hip.HipSolver.from_case(c) as s:
+                s.upload(c["h"], c["hu"], c["hv"], c["B"])
+                want_hist = s.step_adaptive(0.4, 500)
+                want = s.download()
+            got = b.download(i)
+            assert same(hist[:, i], want_hist), (i, int(sides[i]), "dt")
+            assert all(same(x, y) for x, y in zip(got, want)), (i, int(sides[i]))
+    finally:
+        b.close()
