@@ -263,9 +263,21 @@ struct MSlope {
     double w, hu, hv;
 };
 
-__device__ __forceinline__ MSlope slope_of(const MCell &m, const MCell &c, const MCell &p) {
-    return MSlope{minmod(c.w - m.w, p.w - c.w), minmod(c.hu - m.hu, p.hu - c.hu),
-                  minmod(c.hv - m.hv, p.hv - c.hv)};
+// Half the limited slope, 0.5 * minmod_scalar(a, b) (reconstruction/muscl.hpp:8-11; the
+// states are U +- 0.5 * sigma, :16-36): under the modular range guard every operand is 0
+// or at least 2^-152 in magnitude, so the median form of swe_face.h is exact (same bits).
+template <bool SPLIT>
+__device__ __forceinline__ double half_slope(double a, double b) {
+    if constexpr (SPLIT) return half_minmod_split(a, b);
+    return 0.5 * minmod(a, b);
+}
+
+// The half slopes of cell c from its neighbours m and p along x.
+template <bool SPLIT>
+__device__ __forceinline__ MSlope half_slope_of(const MCell &m, const MCell &c, const MCell &p) {
+    return MSlope{half_slope<SPLIT>(c.w - m.w, p.w - c.w),
+                  half_slope<SPLIT>(c.hu - m.hu, p.hu - c.hu),
+                  half_slope<SPLIT>(c.hv - m.hv, p.hv - c.hv)};
 }
 
 template <int STAGE, int RECON, int RIEMANN, bool BATHY>
@@ -324,8 +336,8 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
         constexpr bool SPLIT = decltype(split_tag)::value;
         NT m{a.w, a.hu, a.hv}, p{b.w, b.hu, b.hv};
         if (RECON == kReconMUSCL) {
-            m = NT{a.w + 0.5 * sa.w, a.hu + 0.5 * sa.hu, a.hv + 0.5 * sa.hv};
-            p = NT{b.w - 0.5 * sb.w, b.hu - 0.5 * sb.hu, b.hv - 0.5 * sb.hv};
+            m = NT{a.w + sa.w, a.hu + sa.hu, a.hv + sa.hv};  // sa, sb: half slopes
+            p = NT{b.w - sb.w, b.hu - sb.hu, b.hv - sb.hv};
         }
         fbad = 0;
         const FaceFlux f = face_mod<0, RIEMANN, BATHY, SPLIT>(g, half_g, m, a.b, p, b.b, fbad, 0ull);
@@ -338,8 +350,8 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
     const MCell Pm2 = load(r0 - 2), Pm1 = load(r0 - 1);
     MCell P0 = load(r0), P1 = load(r0 + 1), P2 = load(r0 + 2 <= rmax ? r0 + 2 : rmax);
     MSlope S0 = zero, S1 = zero;
-    if (RECON == kReconMUSCL) S0 = slope_of(Pm1, P0, P1);
-    const MSlope Sm1 = RECON == kReconMUSCL ? slope_of(Pm2, Pm1, P0) : zero;
+    if (RECON == kReconMUSCL) S0 = half_slope_of<false>(Pm1, P0, P1);
+    const MSlope Sm1 = RECON == kReconMUSCL ? half_slope_of<false>(Pm2, Pm1, P0) : zero;
     FaceFlux Fm;
     if constexpr (kSplitScheme) {
         Fm = split ? xface(std::true_type{}, Pm1, Sm1, P0, S0)
@@ -354,7 +366,7 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
                         const MCell &P2, const MSlope &S0, MSlope &S1, const FaceFlux &Fm,
                         FaceFlux &Fx) {
         constexpr bool SPLIT = decltype(split_tag)::value;
-        if (RECON == kReconMUSCL) S1 = slope_of(P0, P1, P2);
+        if (RECON == kReconMUSCL) S1 = half_slope_of<SPLIT>(P0, P1, P2);
         Fx = xface(split_tag, P0, S0, P1, S1);
 
         // y faces: lane j takes face j+1/2 (normal hv, tangential hu).  The lane moves fill
@@ -365,12 +377,13 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
         double pw = from_next_z(P0.w), phv = from_next_z(P0.hv), phu = from_next_z(P0.hu);
         if (RECON == kReconMUSCL) {
             const double dw = pw - P0.w, dv = phv - P0.hv, du = phu - P0.hu;
-            const double sw = minmod(from_prev_z(dw), dw), sv = minmod(from_prev_z(dv), dv);
-            const double su = minmod(from_prev_z(du), du);
-            m = NT{P0.w + 0.5 * sw, P0.hv + 0.5 * sv, P0.hu + 0.5 * su};
-            pw = from_next_z(P0.w - 0.5 * sw);
-            phv = from_next_z(P0.hv - 0.5 * sv);
-            phu = from_next_z(P0.hu - 0.5 * su);
+            const double sw = half_slope<SPLIT>(from_prev_z(dw), dw);
+            const double sv = half_slope<SPLIT>(from_prev_z(dv), dv);
+            const double su = half_slope<SPLIT>(from_prev_z(du), du);
+            m = NT{P0.w + sw, P0.hv + sv, P0.hu + su};
+            pw = from_next_z(P0.w - sw);
+            phv = from_next_z(P0.hv - sv);
+            phu = from_next_z(P0.hu - su);
         }
         const double pb = BATHY ? from_next_z(P0.b) : 0.0;
         fbad = 0;
