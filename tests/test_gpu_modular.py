@@ -191,3 +191,42 @@ def test_rejected_combinations(hip):
     with hip.HipSolver(8, 8, gr.dx, gr.dy, gr.g, hip.BC_TRANSMISSIVE) as s:
         with pytest.raises(hip.SweError):
             s.set_scheme(hip.SCHEME_SERIAL_PLAIN, RECON_PC, RIEMANN_HLL)
+
+
+@pytest.mark.parametrize("seed", range(3))
+@pytest.mark.parametrize("bathy,recon,riemann", SUPPORTED)
+def test_planted_values_sweep(hip, bathy, recon, riemann, seed):
+    """Random shapes around the 60-column strips with the values that decide the split
+    forms, the median slopes and the all-wet votes planted: momenta and bottoms of -0/+0,
+    flat stretches, equal neighbours; bitwise against the oracle (rhs and 3 steps)."""
+    rng = np.random.default_rng(500 + 31 * seed + 7 * recon + riemann + 100 * bathy)
+    nx = int(rng.choice([2, 5, 17, 60, 61, 97]))
+    ny = int(rng.choice([2, 30, 60, 61, 62, 121]))
+    lx, ly = 0.0625 * nx, 0.125 * ny
+    gr, h, hu, hv, B = wet_state(nx, ny, 800 + seed, bathy, lx, ly)
+    pick = lambda frac: rng.random(h.shape) < frac
+    hu[pick(0.05)] = -0.0
+    hv[pick(0.05)] = 0.0
+    hv[pick(0.05)] = -0.0
+    i0, j0 = rng.integers(0, h.shape[0]), rng.integers(0, h.shape[1])
+    h[i0:i0 + 3, :] = h[i0, 0]
+    hu[:, j0:j0 + 4] = hu[0, j0]
+    if bathy:
+        B[pick(0.05)] = -0.0
+        B[i0:i0 + 5, :] = B[i0, 0]
+        oracle.mod_apply_bc(gr, B)
+    oracle.mod_apply_bc(gr, h, hu, hv)
+    with solver(hip, gr, bathy, recon, riemann, int(rng.choice([1, 4, 32]))) as s:
+        s.upload(h, hu, hv, B)
+        got = s.rhs()
+        rc, want = oracle.mod_rhs(gr, bathy, recon, riemann, h.copy(), hu.copy(), hv.copy(), B)
+        assert rc == 0
+        for g_, w_, nm in zip(got, want, ("h", "hu", "hv")):
+            assert same(g_, w_), (nx, ny, "rhs", nm)
+        dt = 0.1 * min(gr.dx, gr.dy)
+        s.upload(h, hu, hv, B)
+        ref = [a.copy() for a in (h, hu, hv)]
+        assert oracle.mod_run(gr, bathy, recon, riemann, *ref, B, 3, dt) == 0
+        s.step(dt, 3)
+        for g_, w_, nm in zip(s.download(), ref, ("h", "hu", "hv")):
+            assert same(g_, w_), (nx, ny, "steps", nm)
