@@ -56,10 +56,13 @@ struct V3 {
 };
 
 // State of one wave's march down its strip, and the per-row step.  The step for output row
-// r sees the cells of rows r (a), r+1 (b), r+2 (n) and loads row r+3 into `nn`; it also
+// r sees the cells of rows r (a) and r+1 (b) and loads row r+2 into `n` first thing, using
+// it only after the row's y faces (which need row r alone) have covered the load; it also
 // takes the slope of row r (sa) and d_{r+1/2} = b - a (d2), and produces the slope of row
 // r+1 (sb) and d_{r+3/2} (d3); fxp is the plus-side flux of face r-1/2, fxn receives that
-// of face r+1/2.  The caller rotates the register names, so nothing is copied.
+// of face r+1/2.  The caller rotates the register names, so nothing is copied.  (Loading
+// row r+3 a row earlier instead holds a fourth row in registers: 150 VGPRs against 142,
+// and the step measured ~1.5% slower.)
 template <int STAGE, int BC, bool SPEED, bool FAST>
 struct March {
     const StageArgs &A;
@@ -93,11 +96,11 @@ struct March {
                   ld(row_ptr(A.u0_hv, ro), jc_bytes)};
     }
 
-    __device__ __forceinline__ void row(int r, const Cell &a, const Cell &b, const Cell &n,
-                                        Cell &nn, const V3 &sa, V3 &sb, const V3 &d2, V3 &d3,
+    __device__ __forceinline__ void row(int r, const Cell &a, const Cell &b, Cell &n,
+                                        const V3 &sa, V3 &sb, const V3 &d2, V3 &d3,
                                         const V3 &fxp, V3 &fxn) {
-        // prefetch row r+3; U^n of row r is loaded now and first used ~500 instructions on
-        nn = load(r + 3 <= rmax ? r + 3 : rmax);
+        // row r+2 and U^n of row r are loaded now and first used after the y faces
+        n = load(r + 2 <= rmax ? r + 2 : rmax);
         V3 u0{0.0, 0.0, 0.0};
         if (STAGE >= 2) u0 = load_u0(r);
         // FAST has its own reciprocal arithmetic; only the exact mode has a split variant.
@@ -119,14 +122,6 @@ struct March {
     __device__ __forceinline__ void row_body(int r, const Cell &a, const Cell &b, const Cell &n,
                                              const V3 &sa, V3 &sb, const V3 &d2, V3 &d3,
                                              const V3 &fxp, V3 &fxn, const V3 &u0) {
-        // ---- x: slope of row r+1, face r+1/2
-        d3 = V3{n.eta - b.eta, n.hu - b.hu, n.hv - b.hv};
-        sb = V3{half_slope<SPLIT>(d2.e, d3.e), half_slope<SPLIT>(d2.u, d3.u),
-                half_slope<SPLIT>(d2.v, d3.v)};
-        const FaceFlux fx = face_flux<FAST, SPLIT>(half_g, g, a.eta + sa.e, a.hu + sa.u, a.hv + sa.v, a.b,
-                                      b.eta - sb.e, b.hu - sb.u, b.hv - sb.v, b.b);
-        fxn = V3{fx.fh, fx.fn_p, fx.ft};
-
         // ---- y: neighbours along the lanes, face j+1/2 in lane j.  The moves fill the lane
         // without a source with 0: lane 63's face, lane 62's (it reads lane 63's slope) and
         // lane 0's slope and divergence belong to halo columns (never stored), and lanes 62
@@ -144,6 +139,15 @@ struct March {
         // y faces: normal momentum is hv, tangential hu
         const FaceFlux fy = face_flux<FAST, SPLIT>(half_g, g, a.eta + sy_e, a.hv + sy_v, a.hu + sy_u, a.b, pe,
                                       pv, pu, pb, kLastLanes);
+
+        // ---- x: slope of row r+1, face r+1/2
+        d3 = V3{n.eta - b.eta, n.hu - b.hu, n.hv - b.hv};
+        sb = V3{half_slope<SPLIT>(d2.e, d3.e), half_slope<SPLIT>(d2.u, d3.u),
+                half_slope<SPLIT>(d2.v, d3.v)};
+        const FaceFlux fx = face_flux<FAST, SPLIT>(half_g, g, a.eta + sa.e, a.hu + sa.u, a.hv + sa.v, a.b,
+                                      b.eta - sb.e, b.hu - sb.u, b.hv - sb.v, b.b);
+        fxn = V3{fx.fh, fx.fn_p, fx.ft};
+
         const double fyp_h = from_prev_z(fy.fh);
         const double fyp_hu = from_prev_z(fy.ft);
         const double fyp_hv = from_prev_z(fy.fn_p);
@@ -188,8 +192,9 @@ struct March {
 };
 
 // Occupancy target of the stage kernel (waves per SIMD); the VGPR budget follows from it
-// (512 / waves, rounded down to the allocation granule: 168 at 3 waves).  Measured on the
-// 8192^2 dam break: 3 waves (no spills) beats 4 (128 VGPRs) by ~2%; 5 and 6 spill.  The
+// (512 / waves, rounded down to the allocation granule: 168 at 3 waves).  The kernel needs
+// up to 146 VGPRs and spills at 4 waves (128); on the 8192^2 dam break 3 waves measured
+// ~2% faster than 4 even for an earlier, smaller version that fitted 4.  The
 // persistent batch kernel has its own target.  Both are overridable at build time for
 // tuning sweeps.
 #ifndef SWE_STAGE_WAVES_PER_EU
@@ -218,11 +223,10 @@ __device__ __forceinline__ void run_tile(const GridDev &G, const StageArgs &A, l
                                     G.half_g, G.g, G.inv_dx, G.inv_dy, dt, speed, bad,
                                     G.split_ok != 0};
 
-    // ---- prologue: rows r0-2 .. r0+2 and the plus-side flux of face r0-1/2
+    // ---- prologue: rows r0-2 .. r0+1 and the plus-side flux of face r0-1/2
     const Cell P2 = m.load(r0 - 2);
     const Cell P1 = m.load(r0 - 1);
-    Cell C0 = m.load(r0), C1 = m.load(r0 + 1), C2 = m.load(r0 + 2 <= m.rmax ? r0 + 2 : m.rmax);
-    Cell C3;
+    Cell C0 = m.load(r0), C1 = m.load(r0 + 1), C2;
 
     // x differences d_{r0-3/2}, d_{r0-1/2}, d_{r0+1/2}
     const V3 dm{P1.eta - P2.eta, P1.hu - P2.hu, P1.hv - P2.hv};
@@ -240,15 +244,20 @@ __device__ __forceinline__ void run_tile(const GridDev &G, const StageArgs &A, l
                                                     C0.hu - S0.u, C0.hv - S0.v, C0.b);
     V3 F0{f0.fh, f0.fn_p, f0.ft}, F1;
 
-    // ---- march, unrolled 4x so the register window rotates by renaming
-    for (int r = r0; r < r1; r += 4) {
-        m.row(r, C0, C1, C2, C3, S0, S1, D0, D1, F0, F1);
+    // ---- march, unrolled 6x (the 3-row window and the 2-deep slope, difference and flux
+    // pairs) so every register window rotates by renaming
+    for (int r = r0; r < r1; r += 6) {
+        m.row(r, C0, C1, C2, S0, S1, D0, D1, F0, F1);
         if (r + 1 >= r1) break;
-        m.row(r + 1, C1, C2, C3, C0, S1, S0, D1, D0, F1, F0);
+        m.row(r + 1, C1, C2, C0, S1, S0, D1, D0, F1, F0);
         if (r + 2 >= r1) break;
-        m.row(r + 2, C2, C3, C0, C1, S0, S1, D0, D1, F0, F1);
+        m.row(r + 2, C2, C0, C1, S0, S1, D0, D1, F0, F1);
         if (r + 3 >= r1) break;
-        m.row(r + 3, C3, C0, C1, C2, S1, S0, D1, D0, F1, F0);
+        m.row(r + 3, C0, C1, C2, S1, S0, D1, D0, F1, F0);
+        if (r + 4 >= r1) break;
+        m.row(r + 4, C1, C2, C0, S0, S1, D0, D1, F0, F1);
+        if (r + 5 >= r1) break;
+        m.row(r + 5, C2, C0, C1, S1, S0, D1, D0, F1, F0);
     }
     speed = m.speed;
     bad = m.bad;
