@@ -35,22 +35,6 @@ struct Cell {
 // First column (lane 0, a halo column) of strip `strip` for ng = 2.
 __device__ __forceinline__ int strip_first_col(int strip) { return strip * kStripCols; }
 
-template <int BC>
-__device__ __forceinline__ Cell load_cell(const StageArgs &A, long long row_off, bool ghost_row,
-                                          unsigned jc_bytes, bool ghost_col) {
-    Cell c;
-    c.h = ld(row_ptr(A.in_h, row_off), jc_bytes);
-    c.hu = ld(row_ptr(A.in_hu, row_off), jc_bytes);
-    c.hv = ld(row_ptr(A.in_hv, row_off), jc_bytes);
-    c.b = ld(row_ptr(A.B, row_off), jc_bytes);
-    if (BC == 0) {
-        if (ghost_row) c.hu = -c.hu;
-        if (ghost_col) c.hv = -c.hv;
-    }
-    c.eta = c.h + c.b;
-    return c;
-}
-
 struct V3 {
     double e, u, v;  // (eta, hu, hv) or, for fluxes, (h, hu, hv) components
 };
@@ -77,15 +61,37 @@ struct March {
     // may use the split IEEE division/sqrt forms (swe_face.h).  Sticky once false.
     bool split;
 
-    __device__ __forceinline__ Cell load(int r) {
+    // Row r as stored (h, hu, hv, b).  Only the loads are issued here: nothing reads the
+    // values before `finish`, so the wave keeps computing while they are in flight.
+    __device__ __forceinline__ Cell load_raw(int r, bool &ghost_row) {
         pin_lane_offset(jc_bytes, r);
-        bool gr;
-        const int rs = mirror<BC>(r, ng, ng + nx, gr);
-        const Cell c = load_cell<BC>(A, base + __builtin_amdgcn_readfirstlane(rs * s), gr,
-                                     jc_bytes, gcol);
+        const int rs = mirror<BC>(r, ng, ng + nx, ghost_row);
+        const long long ro = base + __builtin_amdgcn_readfirstlane(rs * s);
+        Cell c;
+        c.h = ld(row_ptr(A.in_h, ro), jc_bytes);
+        c.hu = ld(row_ptr(A.in_hu, ro), jc_bytes);
+        c.hv = ld(row_ptr(A.in_hv, ro), jc_bytes);
+        c.b = ld(row_ptr(A.B, ro), jc_bytes);
+        return c;
+    }
+
+    // A loaded row made usable: the reflecting ghost sign flips (momentum normal to the
+    // wall negated, omp_solver.hpp:340-393), eta = h + b, and the range guard.
+    __device__ __forceinline__ void finish(Cell &c, bool ghost_row) {
+        if (BC == 0) {
+            if (ghost_row) c.hu = -c.hu;
+            if (gcol) c.hv = -c.hv;
+        }
+        c.eta = c.h + c.b;
         const unsigned long long exec = __builtin_amdgcn_read_exec();
         split = split && (split_ok_lanes(c.h) & split_ok_lanes(c.hu) & split_ok_lanes(c.hv) &
                           split_ok_lanes(c.b) & exec) == exec;
+    }
+
+    __device__ __forceinline__ Cell load(int r) {
+        bool gr;
+        Cell c = load_raw(r, gr);
+        finish(c, gr);
         return c;
     }
 
@@ -99,15 +105,26 @@ struct March {
     __device__ __forceinline__ void row(int r, const Cell &a, const Cell &b, Cell &n,
                                         const V3 &sa, V3 &sb, const V3 &d2, V3 &d3,
                                         const V3 &fxp, V3 &fxn) {
-        // row r+2 and U^n of row r are loaded now and first used after the y faces
-        n = load(r + 2 <= rmax ? r + 2 : rmax);
+        // Row r+2 and U^n of row r are loaded first and not read before the y faces of row
+        // r (which need row r alone, already guarded) are done: the sign flips, eta and the
+        // range guard of row r+2 would otherwise wait for the loads right here.
+        bool gr;
+        n = load_raw(r + 2 <= rmax ? r + 2 : rmax, gr);
         V3 u0{0.0, 0.0, 0.0};
         if (STAGE >= 2) u0 = load_u0(r);
         // FAST has its own reciprocal arithmetic; only the exact mode has a split variant.
+        FaceFlux fy, fx;
         if (!FAST && split)
-            row_body<true>(r, a, b, n, sa, sb, d2, d3, fxp, fxn, u0);
+            fy = y_faces<true>(a);
         else
-            row_body<false>(r, a, b, n, sa, sb, d2, d3, fxp, fxn, u0);
+            fy = y_faces<false>(a);
+        finish(n, gr);
+        d3 = V3{n.eta - b.eta, n.hu - b.hu, n.hv - b.hv};
+        if (!FAST && split)
+            fx = x_face<true>(a, b, sa, sb, d2, d3);
+        else
+            fx = x_face<false>(a, b, sa, sb, d2, d3);
+        tail(r, a, fx, fy, fxp, fxn, u0);
     }
 
     // Half the limited slope, 0.5 * minmod(a, b) (omp_kernels.hpp:146-152, :214-220): the
@@ -118,14 +135,12 @@ struct March {
         return 0.5 * minmod(a, b);
     }
 
+    // y faces of row r: neighbours along the lanes, face j+1/2 in lane j.  The moves fill
+    // the lane without a source with 0: lane 63's face, lane 62's (it reads lane 63's slope)
+    // and lane 0's slope and divergence belong to halo columns (never stored), and lanes 62
+    // and 63 are left out of the face's wave votes.
     template <bool SPLIT>
-    __device__ __forceinline__ void row_body(int r, const Cell &a, const Cell &b, const Cell &n,
-                                             const V3 &sa, V3 &sb, const V3 &d2, V3 &d3,
-                                             const V3 &fxp, V3 &fxn, const V3 &u0) {
-        // ---- y: neighbours along the lanes, face j+1/2 in lane j.  The moves fill the lane
-        // without a source with 0: lane 63's face, lane 62's (it reads lane 63's slope) and
-        // lane 0's slope and divergence belong to halo columns (never stored), and lanes 62
-        // and 63 are left out of the face's wave votes.
+    __device__ __forceinline__ FaceFlux y_faces(const Cell &a) const {
         const double dy_e = from_next_z(a.eta) - a.eta;
         const double dy_u = from_next_z(a.hu) - a.hu;
         const double dy_v = from_next_z(a.hv) - a.hv;
@@ -136,16 +151,26 @@ struct March {
         const double pu = from_next_z(a.hu - sy_u);
         const double pv = from_next_z(a.hv - sy_v);
         const double pb = from_next_z(a.b);
-        // y faces: normal momentum is hv, tangential hu
-        const FaceFlux fy = face_flux<FAST, SPLIT>(half_g, g, a.eta + sy_e, a.hv + sy_v, a.hu + sy_u, a.b, pe,
-                                      pv, pu, pb, kLastLanes);
+        // normal momentum is hv, tangential hu
+        return face_flux<FAST, SPLIT>(half_g, g, a.eta + sy_e, a.hv + sy_v, a.hu + sy_u, a.b,
+                                      pe, pv, pu, pb, kLastLanes);
+    }
 
-        // ---- x: slope of row r+1, face r+1/2
-        d3 = V3{n.eta - b.eta, n.hu - b.hu, n.hv - b.hv};
+    // x face r+1/2, with the slope of row r+1 from d2 = d_{r+1/2} and d3 = d_{r+3/2}.
+    template <bool SPLIT>
+    __device__ __forceinline__ FaceFlux x_face(const Cell &a, const Cell &b, const V3 &sa,
+                                               V3 &sb, const V3 &d2, const V3 &d3) const {
         sb = V3{half_slope<SPLIT>(d2.e, d3.e), half_slope<SPLIT>(d2.u, d3.u),
                 half_slope<SPLIT>(d2.v, d3.v)};
-        const FaceFlux fx = face_flux<FAST, SPLIT>(half_g, g, a.eta + sa.e, a.hu + sa.u, a.hv + sa.v, a.b,
+        return face_flux<FAST, SPLIT>(half_g, g, a.eta + sa.e, a.hu + sa.u, a.hv + sa.v, a.b,
                                       b.eta - sb.e, b.hu - sb.u, b.hv - sb.v, b.b);
+    }
+
+    // Divergence of the row's four faces, the stage combination, positivity floor,
+    // non-finite flag and wave speed, and the stores of row r.
+    __device__ __forceinline__ void tail(int r, const Cell &a, const FaceFlux &fx,
+                                         const FaceFlux &fy, const V3 &fxp, V3 &fxn,
+                                         const V3 &u0) {
         fxn = V3{fx.fh, fx.fn_p, fx.ft};
 
         const double fyp_h = from_prev_z(fy.fh);
@@ -157,20 +182,29 @@ struct March {
         const double rhu = divergence(inv_dx, inv_dy, fx.fn_m, fxp.u, fy.ft, fyp_hu);
         const double rhv = divergence(inv_dx, inv_dy, fx.ft, fxp.v, fy.fn_m, fyp_hv);
 
-        if (owner) {
-            const long long ro = base + __builtin_amdgcn_readfirstlane(r * s);
-            if (STAGE == 0) {
+        const long long ro = base + __builtin_amdgcn_readfirstlane(r * s);
+        if (STAGE == 0) {
+            if (owner) {
                 pin_lane_offset(j_bytes, r);
                 st(row_ptr(A.out_h, ro), j_bytes, rh);
                 st(row_ptr(A.out_hu, ro), j_bytes, rhu);
                 st(row_ptr(A.out_hv, ro), j_bytes, rhv);
-            } else {
-                const double s0h = STAGE == 1 ? a.h : u0.e;
-                const double s0u = STAGE == 1 ? a.hu : u0.u;
-                const double s0v = STAGE == 1 ? a.hv : u0.v;
-                double hN = rk_combine<STAGE>(s0h, a.h, dt, rh);
-                double huN = rk_combine<STAGE>(s0u, a.hu, dt, rhu);
-                double hvN = rk_combine<STAGE>(s0v, a.hv, dt, rhv);
+            }
+        } else {
+            // U^n is read here on every path through the row.  The combination below is used
+            // only in the owner branch, which the compiler may skip when no lane owns a
+            // column, and it sinks the combination into it; on that path the U^n loads would
+            // still be in flight when the next row starts, and the compiler would wait for
+            // them there -- and, the vector memory counter being in order, for this row's
+            // stores as well -- before reusing their registers.
+            if (STAGE >= 2) asm volatile("" ::"v"(u0.e), "v"(u0.u), "v"(u0.v));
+            const double s0h = STAGE == 1 ? a.h : u0.e;
+            const double s0u = STAGE == 1 ? a.hu : u0.u;
+            const double s0v = STAGE == 1 ? a.hv : u0.v;
+            double hN = rk_combine<STAGE>(s0h, a.h, dt, rh);
+            double huN = rk_combine<STAGE>(s0u, a.hu, dt, rhu);
+            double hvN = rk_combine<STAGE>(s0v, a.hv, dt, rhv);
+            if (owner) {
                 // enforce_positivity (omp_kernels.hpp:317-325); the selects run only when some
                 // lane of the wave is below the floor (same values)
                 if (wave_any(hN < kFloor)) {
