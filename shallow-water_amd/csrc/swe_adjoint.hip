@@ -324,44 +324,61 @@ struct Lanes {
         return (long long)__builtin_amdgcn_readfirstlane(r * s);
     }
 
-    // Row r of V (r within 2 of the interior); `floor` = the stored cell is the floor state.
-    __device__ __forceinline__ PCell load(int r, bool &floor) const {
+    // Row r of V (r within 2 of the interior) as stored, h in the eta slot.  Only the loads
+    // are issued here: nothing reads the values before `finish`, so the wave keeps
+    // computing while they are in flight.
+    __device__ __forceinline__ PCell load_raw(int r, bool &ghost_row) const {
         pin_lane_offset(jc_bytes, r);
-        bool gr;
-        const int rs = mirror<BC>(r, ng, ng + nx, gr);
+        const int rs = mirror<BC>(r, ng, ng + nx, ghost_row);
         const long long ro = row_off(rs);
         PCell c;
-        const double h = ld(row_ptr(A.v_h, ro), jc_bytes);
+        c.eta = ld(row_ptr(A.v_h, ro), jc_bytes);
         c.hu = ld(row_ptr(A.v_hu, ro), jc_bytes);
         c.hv = ld(row_ptr(A.v_hv, ro), jc_bytes);
         c.b = ld(row_ptr(A.B, ro), jc_bytes);
+        return c;
+    }
+    // A loaded row made usable: `floor` = the stored cell is the floor state; the reflecting
+    // sign flips; eta = h + b.
+    __device__ __forceinline__ void finish(PCell &c, bool ghost_row, bool &floor) const {
+        const double h = c.eta;
         floor = floor_state(h, c.hu, c.hv);
         if (BC == 0) {
-            if (gr) c.hu = -c.hu;
+            if (ghost_row) c.hu = -c.hu;
             if (gcol) c.hv = -c.hv;
         }
         c.eta = h + c.b;
+    }
+    __device__ __forceinline__ PCell load(int r, bool &floor) const {
+        bool gr;
+        PCell c = load_raw(r, gr);
+        finish(c, gr, floor);
         return c;
     }
 
-    // lamW of row r: the output adjoint where the output is not the floor state, 0 outside
-    // the interior.
-    __device__ __forceinline__ D3 load_lam(int r) const {
-        D3 l{0.0, 0.0, 0.0};
-        if (r < ng || r >= ng + nx) return l;  // wave-uniform
+    // lamW of row r as stored (r in [ng-2, ng+nx+1], inside the padded layout), with the
+    // output's floor flag.  Every lane loads, at the source column (its own column where
+    // xact, in bounds everywhere), and rows outside the interior load too: no branch, so
+    // nothing forces the values to be read before `finish_lam`.
+    struct LamRaw {
+        D3 l;
+        unsigned m;
+        bool row_in;  // wave-uniform: r is an interior row
+    };
+    __device__ __forceinline__ LamRaw load_lam_raw(int r) const {
         const long long ro = row_off(r);
-        if (xact) {
-            pin_lane_offset(j_bytes, r);
-            const bool m = A.mask_out[ro + (j_bytes >> 3)] != 0;
-            const double a = ld(row_ptr(A.lam_h, ro), j_bytes);
-            const double b = ld(row_ptr(A.lam_hu, ro), j_bytes);
-            const double c = ld(row_ptr(A.lam_hv, ro), j_bytes);
-            l.e = m ? 0.0 : a;
-            l.u = m ? 0.0 : b;
-            l.v = m ? 0.0 : c;
-        }
-        return l;
+        pin_lane_offset(jc_bytes, r);
+        return LamRaw{D3{ld(row_ptr(A.lam_h, ro), jc_bytes), ld(row_ptr(A.lam_hu, ro), jc_bytes),
+                         ld(row_ptr(A.lam_hv, ro), jc_bytes)},
+                      A.mask_out[ro + (jc_bytes >> 3)], r >= ng && r < ng + nx};
     }
+    // lamW: the output adjoint where the output is not the floor state, 0 outside the
+    // interior rows and columns.
+    __device__ __forceinline__ D3 finish_lam(const LamRaw &q) const {
+        const bool z = !q.row_in || !xact || q.m != 0;
+        return D3{z ? 0.0 : q.l.e, z ? 0.0 : q.l.u, z ? 0.0 : q.l.v};
+    }
+    __device__ __forceinline__ D3 load_lam(int r) const { return finish_lam(load_lam_raw(r)); }
 
     // The lanes whose cell is inside the split forms' range guard (depth h = eta - b,
     // momenta, bottom), one ballot per compare.
@@ -544,11 +561,42 @@ __device__ __forceinline__ A4 gather_folded(const S &X, const GridDev &G, int r)
     return T;
 }
 
+// The values emit_cell reads back at row r: the U^n accumulator (stages 2, 1) and dL/dB,
+// loaded at the source column (the own column for owner lanes, in bounds for every lane)
+// so that they can be issued at the start of the row.
+struct EmitIn {
+    D3 e;
+    double lb;
+};
+template <int STAGE, int BC>
+__device__ __forceinline__ EmitIn load_emit(const AdjArgs &A, const Lanes<BC> &w, int r) {
+    const long long ro = w.row_off(r);
+    unsigned jc = w.jc_bytes;
+    pin_lane_offset(jc, r);
+    EmitIn E{{0.0, 0.0, 0.0}, 0.0};
+    if constexpr (STAGE != 3)
+        E.e = D3{ld(row_ptr(A.e_h, ro), jc), ld(row_ptr(A.e_hu, ro), jc),
+                 ld(row_ptr(A.e_hv, ro), jc)};
+    E.lb = ld(row_ptr(A.lam_b, ro), jc);
+    return E;
+}
+// Reads the early loads on every path through the row: emit_cell runs only in the owner
+// branch, which the compiler may skip when no lane owns a column, and loads still in flight
+// at the next row would make it wait there -- and, the vector memory counter being in
+// order, for this row's stores as well.
+template <int STAGE>
+__device__ __forceinline__ void use_emit(const EmitIn &E) {
+    if constexpr (STAGE != 3) asm volatile("" ::"v"(E.e.e), "v"(E.e.u), "v"(E.e.v));
+    asm volatile("" ::"v"(E.lb));
+}
+
 // The stage's outputs at an owned cell of row r: dL/dV (stage 1: plus the U^n accumulator),
-// the U^n accumulator, dL/dB.  acc = the y faces' part, T = the x faces' part.
+// the U^n accumulator, dL/dB.  acc = the y faces' part, T = the x faces' part, E = the
+// accumulator and dL/dB as they were before this stage.
 template <int STAGE, int BC>
 __device__ __forceinline__ void emit_cell(const AdjArgs &A, const Lanes<BC> &w, int r,
-                                          const D3 &L, const A4 &acc, const A4 &T) {
+                                          const D3 &L, const A4 &acc, const A4 &T,
+                                          const EmitIn &E) {
     constexpr double cv = STAGE == 1 ? 1.0 : (STAGE == 2 ? 0.25 : 2.0 / 3.0);
     const long long ro = w.row_off(r);
     unsigned jb = w.j_bytes;
@@ -561,18 +609,18 @@ __device__ __forceinline__ void emit_cell(const AdjArgs &A, const Lanes<BC> &w, 
         st(row_ptr(A.e_hu, ro), jb, (1.0 / 3.0) * L.u);
         st(row_ptr(A.e_hv, ro), jb, (1.0 / 3.0) * L.v);
     } else if constexpr (STAGE == 2) {
-        st(row_ptr(A.e_h, ro), jb, ld(row_ptr(A.e_h, ro), jb) + 0.75 * L.e);
-        st(row_ptr(A.e_hu, ro), jb, ld(row_ptr(A.e_hu, ro), jb) + 0.75 * L.u);
-        st(row_ptr(A.e_hv, ro), jb, ld(row_ptr(A.e_hv, ro), jb) + 0.75 * L.v);
+        st(row_ptr(A.e_h, ro), jb, E.e.e + 0.75 * L.e);
+        st(row_ptr(A.e_hu, ro), jb, E.e.u + 0.75 * L.u);
+        st(row_ptr(A.e_hv, ro), jb, E.e.v + 0.75 * L.v);
     } else {
-        oh += ld(row_ptr(A.e_h, ro), jb);
-        ou += ld(row_ptr(A.e_hu, ro), jb);
-        ov += ld(row_ptr(A.e_hv, ro), jb);
+        oh += E.e.e;
+        ou += E.e.u;
+        ov += E.e.v;
     }
     st(row_ptr(A.out_h, ro), jb, oh);
     st(row_ptr(A.out_hu, ro), jb, ou);
     st(row_ptr(A.out_hv, ro), jb, ov);
-    st(row_ptr(A.lam_b, ro), jb, ld(row_ptr(A.lam_b, ro), jb) + ((T.e + T.b) + (acc.e + acc.b)));
+    st(row_ptr(A.lam_b, ro), jb, E.lb + ((T.e + T.b) + (acc.e + acc.b)));
 }
 
 // ---- y faces, x gather and the stage's outputs: every row independently -----------------
@@ -599,15 +647,21 @@ void adjoint_y_kernel(const GridDev G, const AdjArgs A) {
     D3 L = w.load_lam(r0);
 #pragma unroll 1
     for (int r = r0; r < r1; ++r) {
+        // the next row, its lamW and this row's read-back values are loaded first and
+        // finished after the row's arithmetic
         const int rn = r + 1 < r1 ? r + 1 : r;
-        const PCell Pn = w.load(rn, fl);
-        const D3 Ln = w.load_lam(rn);
+        bool gr;
+        PCell Pn = w.load_raw(rn, gr);
+        const auto Lnr = w.load_lam_raw(rn);
+        const EmitIn E = load_emit<STAGE, BC>(A, w, r);
         A4 acc = (G.split_ok && Lanes<BC>::all_lanes(Lanes<BC>::split_lanes(P)))
                      ? w.template y_row<true>(G.half_g, G.g, ky, P, L)
                      : w.template y_row<false>(G.half_g, G.g, ky, P, L);
-        if (w.owner) emit_cell<STAGE, BC>(A, w, r, L, acc, gather_folded<BC>(X, G, r));
+        use_emit<STAGE>(E);
+        if (w.owner) emit_cell<STAGE, BC>(A, w, r, L, acc, gather_folded<BC>(X, G, r), E);
+        w.finish(Pn, gr, fl);
         P = Pn;
-        L = Ln;
+        L = w.finish_lam(Lnr);
     }
 }
 
@@ -664,10 +718,11 @@ void adjoint_xface_kernel(const GridDev G, const AdjArgs A) {
 #pragma unroll 1
     for (int f = f0; f < f1; ++f) {
         // next face's row of V (written to the floor mask when owned) and lamW
+        // (loaded now, finished after the face)
         const int rn = f + 3 <= ng + nx + 1 ? f + 3 : ng + nx + 1;
-        P3 = w.load(rn, fl);
-        if (rn == f + 3) mark(rn, fl);
-        L2 = w.load_lam(f + 2);
+        bool gr3;
+        P3 = w.load_raw(rn, gr3);
+        const auto L2r = w.load_lam_raw(f + 2);
 
         int sa, sb, sc;
         S1.e = half_minmod(P1.eta - P0.eta, P2.eta - P1.eta, sa);
@@ -702,6 +757,9 @@ void adjoint_xface_kernel(const GridDev G, const AdjArgs A) {
             sr[w.j_bytes >> 3] = (unsigned char)sel0;
             if (f == ng + nx - 1) sr[w.s + (w.j_bytes >> 3)] = (unsigned char)sel1;
         }
+        w.finish(P3, gr3, fl);
+        if (rn == f + 3) mark(rn, fl);
+        L2 = w.finish_lam(L2r);
         split = split && Lanes<BC>::all_lanes(Lanes<BC>::split_lanes(P3));
         P0 = P1;
         P1 = P2;
@@ -767,9 +825,11 @@ void adjoint_stage_kernel(const GridDev G, const AdjArgs A) {
             D3 L0 = w.load_lam(fa), L1 = w.load_lam(fa + 1), L2;
 #pragma unroll 1
             for (int f = fa; f < fb; ++f) {
+                // row f+3 and lamW of row f+2: loaded now, finished after the face
                 const int rn = f + 3 <= ng + nx + 1 ? f + 3 : ng + nx + 1;
-                P3 = w.load(rn, fl);
-                L2 = w.load_lam(f + 2);
+                bool gr3;
+                P3 = w.load_raw(rn, gr3);
+                const auto L2r = w.load_lam_raw(f + 2);
                 int sa, sb, sc;
                 S1.e = half_minmod(P1.eta - P0.eta, P2.eta - P1.eta, sa);
                 S1.u = half_minmod(P1.hu - P0.hu, P2.hu - P1.hu, sb);
@@ -798,6 +858,8 @@ void adjoint_stage_kernel(const GridDev G, const AdjArgs A) {
                 slot[7 * kWave] = x.bp;
                 lds_sel[(f - F0) * kWave + lane] = (unsigned char)sel0;
                 if (f == F1) lds_sel[(f + 1 - F0) * kWave + lane] = (unsigned char)sel1;
+                w.finish(P3, gr3, fl);
+                L2 = w.finish_lam(L2r);
                 split = split && Lanes<BC>::all_lanes(Lanes<BC>::split_lanes(P3));
                 P0 = P1;
                 P1 = P2;
@@ -822,18 +884,24 @@ void adjoint_stage_kernel(const GridDev G, const AdjArgs A) {
     D3 L = w.load_lam(ra);
 #pragma unroll 1
     for (int r = ra; r < rb; ++r) {
+        // the next row, its lamW and this row's read-back values are loaded first and
+        // finished after the row's arithmetic
         const int rn = r + 1 < rb ? r + 1 : r;
-        const PCell Pn = w.load(rn, fln);
-        const D3 Ln = w.load_lam(rn);
+        bool grn;
+        PCell Pn = w.load_raw(rn, grn);
+        const auto Lnr = w.load_lam_raw(rn);
+        const EmitIn E = load_emit<STAGE, BC>(A, w, r);
         A4 acc = (G.split_ok && Lanes<BC>::all_lanes(Lanes<BC>::split_lanes(P)))
                      ? w.template y_row<true>(G.half_g, G.g, ky, P, L)
                      : w.template y_row<false>(G.half_g, G.g, ky, P, L);
+        use_emit<STAGE>(E);
         if (w.owner) {
             A.mask_in[w.row_off(r) + (w.j_bytes >> 3)] = fl ? 1 : 0;
-            emit_cell<STAGE, BC>(A, w, r, L, acc, gather_folded<BC>(X, G, r));
+            emit_cell<STAGE, BC>(A, w, r, L, acc, gather_folded<BC>(X, G, r), E);
         }
+        w.finish(Pn, grn, fln);
         P = Pn;
-        L = Ln;
+        L = w.finish_lam(Lnr);
         fl = fln;
     }
 }

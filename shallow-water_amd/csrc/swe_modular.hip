@@ -308,18 +308,24 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
     constexpr bool kSplitScheme = RIEMANN != kRusanov;
     bool split = kSplitScheme && G.split_ok != 0;
 
-    auto load = [&](int r) {
+    // Row r as stored, with h in the w slot.  Only the loads are issued here: nothing reads
+    // the values before `finish`, so the wave keeps computing while they are in flight.
+    auto load_raw = [&](int r, bool &gr) {
         pin_lane_offset(jcb, r);
-        bool gr;
         const int rs = mirror<0>(r, ng, ng + nx, gr);
         const long long ro = __builtin_amdgcn_readfirstlane(rs * s);
         MCell c;
-        const double h = ld(row_ptr(A.in_h, ro), jcb);
+        c.w = ld(row_ptr(A.in_h, ro), jcb);
         c.hu = ld(row_ptr(A.in_hu, ro), jcb);
         c.hv = ld(row_ptr(A.in_hv, ro), jcb);
+        c.b = BATHY ? ld(row_ptr(A.B, ro), jcb) : 0.0;
+        return c;
+    };
+    // A loaded row made usable: the reflecting sign flips, w = h + b, and the range guard.
+    auto finish = [&](MCell &c, bool gr) {
+        const double h = c.w;
         if (gr) c.hu = -c.hu;
         if (gcol) c.hv = -c.hv;
-        c.b = BATHY ? ld(row_ptr(A.B, ro), jcb) : 0.0;
         c.w = BATHY ? h + c.b : h;
         if constexpr (kSplitScheme) {
             unsigned long long ok = mod_in_range(h) & mod_zero_or_in_range(c.hu) &
@@ -328,6 +334,11 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
             const unsigned long long exec = __builtin_amdgcn_read_exec();
             split = split && (ok & exec) == exec;
         }
+    };
+    auto load = [&](int r) {
+        bool gr;
+        MCell c = load_raw(r, gr);
+        finish(c, gr);
         return c;
     };
     // x face between rows a (minus, slope sa) and b (plus, slope sb): normal hu
@@ -364,7 +375,7 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
     // the slope of row r, s1 receives that of row r+1; fm is face r-1/2, fx receives r+1/2.
     auto row_body = [&](auto split_tag, int r, const MCell &P0, const MCell &P1,
                         const MCell &P2, const MSlope &S0, MSlope &S1, const FaceFlux &Fm,
-                        FaceFlux &Fx) {
+                        FaceFlux &Fx, const NT &u0, double in_h) {
         constexpr bool SPLIT = decltype(split_tag)::value;
         if (RECON == kReconMUSCL) S1 = half_slope_of<SPLIT>(P0, P1, P2);
         Fx = xface(split_tag, P0, S0, P1, S1);
@@ -399,6 +410,12 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
         const double rhu = divergence(G.inv_dx, G.inv_dy, Fx.fn_m, Fm.fn_p, Fy.ft, fyp_hu);
         const double rhv = divergence(G.inv_dx, G.inv_dy, Fx.ft, Fm.ft, Fy.fn_m, fyp_hv);
 
+        // The early loads are read here on every path: the stage combination is used only
+        // in the owner branch, which the compiler may skip when no lane owns a column, and
+        // loads still in flight at the next row would make it wait there for them -- and,
+        // the vector memory counter being in order, for this row's stores as well.
+        if constexpr (STAGE >= 2) asm volatile("" ::"v"(u0.h), "v"(u0.n), "v"(u0.t));
+        if constexpr (STAGE >= 1 && BATHY) asm volatile("" ::"v"(in_h));
         if (owner) {
             const long long ro = __builtin_amdgcn_readfirstlane(r * s);
             pin_lane_offset(jb, r);
@@ -407,23 +424,23 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
                 st(row_ptr(A.out_hu, ro), jb, rhu);
                 st(row_ptr(A.out_hv, ro), jb, rhv);
             } else {
-                // the stage input is re-read (cache hit): with bathymetry P0.w is eta
+                // The stage input U_s at an owner cell (interior, never sign-flipped): P0's hu
+                // and hv, and h = P0.w without bathymetry; with bathymetry P0.w is eta and h
+                // was loaded at the start of the row.
+                const double sh = BATHY ? in_h : P0.w;
                 double hN, huN, hvN;
                 if constexpr (STAGE == 1) {
                     // ssp_rk3.hpp:10-15 (1.0 * U + dt * L)
-                    hN = 1.0 * ld(row_ptr(A.in_h, ro), jb) + dt * rh;
-                    huN = 1.0 * ld(row_ptr(A.in_hu, ro), jb) + dt * rhu;
-                    hvN = 1.0 * ld(row_ptr(A.in_hv, ro), jb) + dt * rhv;
+                    hN = 1.0 * sh + dt * rh;
+                    huN = 1.0 * P0.hu + dt * rhu;
+                    hvN = 1.0 * P0.hv + dt * rhv;
                 } else {
                     const double a = STAGE == 2 ? 3.0 / 4.0 : 1.0 / 3.0;
                     const double b = STAGE == 2 ? 1.0 / 4.0 : 2.0 / 3.0;
                     const double c = STAGE == 2 ? dt / 4.0 : 2.0 * dt / 3.0;
-                    hN = a * ld(row_ptr(A.u0_h, ro), jb) + b * ld(row_ptr(A.in_h, ro), jb) +
-                         c * rh;
-                    huN = a * ld(row_ptr(A.u0_hu, ro), jb) + b * ld(row_ptr(A.in_hu, ro), jb) +
-                          c * rhu;
-                    hvN = a * ld(row_ptr(A.u0_hv, ro), jb) + b * ld(row_ptr(A.in_hv, ro), jb) +
-                          c * rhv;
+                    hN = a * u0.h + b * sh + c * rh;
+                    huN = a * u0.n + b * P0.hu + c * rhu;
+                    hvN = a * u0.t + b * P0.hv + c * rhv;
                 }
                 if constexpr (BATHY) {
                     // serial_solver.hpp:453-472
@@ -443,15 +460,32 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
     };
     auto row = [&](int r, const MCell &P0, const MCell &P1, const MCell &P2, MCell &Pn,
                    const MSlope &S0, MSlope &S1, const FaceFlux &Fm, FaceFlux &Fx) {
-        Pn = load(r + 3 <= rmax ? r + 3 : rmax);
+        // row r+3 is first used by the next row: it is finished (and guarded) after this
+        // row's faces, a row after its loads were issued
+        bool gr;
+        Pn = load_raw(r + 3 <= rmax ? r + 3 : rmax, gr);
+        // U^n of row r (stages 2, 3) and, with bathymetry, h of row r, for the stage
+        // combination: issued now, at the source column (the own column for owner lanes,
+        // in bounds for every lane), and read at the end of the row
+        NT u0{0.0, 0.0, 0.0};
+        double in_h = 0.0;
+        if constexpr (STAGE >= 2 || (STAGE == 1 && BATHY)) {
+            pin_lane_offset(jcb, r);
+            const long long ro = __builtin_amdgcn_readfirstlane(r * s);
+            if constexpr (STAGE >= 2)
+                u0 = NT{ld(row_ptr(A.u0_h, ro), jcb), ld(row_ptr(A.u0_hu, ro), jcb),
+                        ld(row_ptr(A.u0_hv, ro), jcb)};
+            if constexpr (BATHY) in_h = ld(row_ptr(A.in_h, ro), jcb);
+        }
         if constexpr (kSplitScheme) {
             if (split)
-                row_body(std::true_type{}, r, P0, P1, P2, S0, S1, Fm, Fx);
+                row_body(std::true_type{}, r, P0, P1, P2, S0, S1, Fm, Fx, u0, in_h);
             else
-                row_body(std::false_type{}, r, P0, P1, P2, S0, S1, Fm, Fx);
+                row_body(std::false_type{}, r, P0, P1, P2, S0, S1, Fm, Fx, u0, in_h);
         } else {
-            row_body(std::false_type{}, r, P0, P1, P2, S0, S1, Fm, Fx);
+            row_body(std::false_type{}, r, P0, P1, P2, S0, S1, Fm, Fx, u0, in_h);
         }
+        finish(Pn, gr);
     };
 
     // march, unrolled 4x so the register window rotates by renaming instead of copies
