@@ -45,7 +45,7 @@ struct V3 {
 // takes the slope of row r (sa) and d_{r+1/2} = b - a (d2), and produces the slope of row
 // r+1 (sb) and d_{r+3/2} (d3); fxp is the plus-side flux of face r-1/2, fxn receives that
 // of face r+1/2.  The caller rotates the register names, so nothing is copied.  (Loading
-// row r+3 a row earlier instead holds a fourth row in registers: 150 VGPRs against 142,
+// row r+3 a row earlier instead holds a fourth row in registers: 148 VGPRs against 140,
 // and the step measured ~1.5% slower.)
 template <int STAGE, int BC, bool SPEED, bool FAST>
 struct March {
