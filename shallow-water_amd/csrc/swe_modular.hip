@@ -371,11 +371,10 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
         Fm = xface(std::false_type{}, Pm1, Sm1, P0, S0);
     }
 
-    // One output row: rows r (c0), r+1 (c1), r+2 (c2) are loaded, row r+3 goes into cn; s0 is
-    // the slope of row r, s1 receives that of row r+1; fm is face r-1/2, fx receives r+1/2.
-    auto row_body = [&](auto split_tag, int r, const MCell &P0, const MCell &P1,
-                        const MCell &P2, const MSlope &S0, MSlope &S1, const FaceFlux &Fm,
-                        FaceFlux &Fx, const NT &u0, double in_h) {
+    // The row's faces (the split and IEEE variants): the slope of row r+1, x face r+1/2 and
+    // the y faces of row r.
+    auto faces = [&](auto split_tag, const MCell &P0, const MCell &P1, const MCell &P2,
+                     const MSlope &S0, MSlope &S1, FaceFlux &Fx, FaceFlux &Fy) {
         constexpr bool SPLIT = decltype(split_tag)::value;
         if (RECON == kReconMUSCL) S1 = half_slope_of<SPLIT>(P0, P1, P2);
         Fx = xface(split_tag, P0, S0, P1, S1);
@@ -398,10 +397,15 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
         }
         const double pb = BATHY ? from_next_z(P0.b) : 0.0;
         fbad = 0;
-        const FaceFlux Fy = face_mod<1, RIEMANN, BATHY, SPLIT>(g, half_g, m, P0.b,
-                                                               NT{pw, phv, phu}, pb, fbad,
-                                                               kLastLanes);
+        Fy = face_mod<1, RIEMANN, BATHY, SPLIT>(g, half_g, m, P0.b, NT{pw, phv, phu}, pb, fbad,
+                                                kLastLanes);
         bad |= yface ? fbad : 0;
+    };
+    // The rest of the row, one copy for both face variants (two copies would be merged by
+    // the compiler at their common stores, which then take a per-lane 64-bit address):
+    // divergence, stage combination, floor, non-finite flag, stores.
+    auto tail = [&](int r, const MCell &P0, const FaceFlux &Fm, const FaceFlux &Fx,
+                    const FaceFlux &Fy, const NT &u0, double in_h) {
         const double fyp_h = from_prev_z(Fy.fh), fyp_hu = from_prev_z(Fy.ft);
         const double fyp_hv = from_prev_z(Fy.fn_p);
 
@@ -458,6 +462,8 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
             }
         }
     };
+    // One output row: rows r (P0), r+1 (P1), r+2 (P2) are loaded, row r+3 goes into Pn; S0 is
+    // the slope of row r, S1 receives that of row r+1; Fm is face r-1/2, Fx receives r+1/2.
     auto row = [&](int r, const MCell &P0, const MCell &P1, const MCell &P2, MCell &Pn,
                    const MSlope &S0, MSlope &S1, const FaceFlux &Fm, FaceFlux &Fx) {
         // row r+3 is first used by the next row: it is finished (and guarded) after this
@@ -477,14 +483,16 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
                         ld(row_ptr(A.u0_hv, ro), jcb)};
             if constexpr (BATHY) in_h = ld(row_ptr(A.in_h, ro), jcb);
         }
+        FaceFlux Fy;
         if constexpr (kSplitScheme) {
             if (split)
-                row_body(std::true_type{}, r, P0, P1, P2, S0, S1, Fm, Fx, u0, in_h);
+                faces(std::true_type{}, P0, P1, P2, S0, S1, Fx, Fy);
             else
-                row_body(std::false_type{}, r, P0, P1, P2, S0, S1, Fm, Fx, u0, in_h);
+                faces(std::false_type{}, P0, P1, P2, S0, S1, Fx, Fy);
         } else {
-            row_body(std::false_type{}, r, P0, P1, P2, S0, S1, Fm, Fx, u0, in_h);
+            faces(std::false_type{}, P0, P1, P2, S0, S1, Fx, Fy);
         }
+        tail(r, P0, Fm, Fx, Fy, u0, in_h);
         finish(Pn, gr);
     };
 
