@@ -194,7 +194,7 @@ __device__ __forceinline__ NT roe(double g, double half_g, const NT &L, const NT
               0.5 * (FL.t + FR.t) - st_ * 0.5};
 }
 
-// The selected Riemann solver (Rusanov always with the IEEE sequences).
+// The selected Riemann solver.
 template <int DIR, int RIEMANN, bool SPLIT, bool WET>
 __device__ __forceinline__ NT riemann_v(double g, double half_g, const NT &L, const NT &R,
                                         int &bad) {
@@ -301,12 +301,11 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
     const double dt = STAGE != 0 ? *A.dt : 0.0;
     int bad = 0;   // HLL without dissipation speed, or a non-finite stage output
     int fbad = 0;  // per face, then masked
-    // Wave-uniform, sticky: every cell loaded so far by this wave passes the modular range
-    // guard, so its faces may use the split division and sqrt forms.  Rusanov has too few
-    // divisions for the split forms to pay for the guard (measured 4% slower with
-    // piecewise-constant reconstruction): it keeps the IEEE sequences.
-    constexpr bool kSplitScheme = RIEMANN != kRusanov;
-    bool split = kSplitScheme && G.split_ok != 0;
+    // Wave-uniform, sticky: every cell finished so far by this wave passes the modular range
+    // guard, so its faces may use the split division and sqrt forms (all three Riemann
+    // solvers: with the guard evaluated off the load path, MUSCL + Rusanov measured 7.5%
+    // faster with the split forms and piecewise-constant Rusanov the same).
+    bool split = G.split_ok != 0;
 
     // Row r as stored, with h in the w slot.  Only the loads are issued here: nothing reads
     // the values before `finish`, so the wave keeps computing while they are in flight.
@@ -327,13 +326,11 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
         if (gr) c.hu = -c.hu;
         if (gcol) c.hv = -c.hv;
         c.w = BATHY ? h + c.b : h;
-        if constexpr (kSplitScheme) {
-            unsigned long long ok = mod_in_range(h) & mod_zero_or_in_range(c.hu) &
-                                    mod_zero_or_in_range(c.hv);
-            if (BATHY) ok &= mod_zero_or_in_range(c.b);
-            const unsigned long long exec = __builtin_amdgcn_read_exec();
-            split = split && (ok & exec) == exec;
-        }
+        unsigned long long ok = mod_in_range(h) & mod_zero_or_in_range(c.hu) &
+                                mod_zero_or_in_range(c.hv);
+        if (BATHY) ok &= mod_zero_or_in_range(c.b);
+        const unsigned long long exec = __builtin_amdgcn_read_exec();
+        split = split && (ok & exec) == exec;
     };
     auto load = [&](int r) {
         bool gr;
@@ -364,12 +361,8 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
     if (RECON == kReconMUSCL) S0 = half_slope_of<false>(Pm1, P0, P1);
     const MSlope Sm1 = RECON == kReconMUSCL ? half_slope_of<false>(Pm2, Pm1, P0) : zero;
     FaceFlux Fm;
-    if constexpr (kSplitScheme) {
-        Fm = split ? xface(std::true_type{}, Pm1, Sm1, P0, S0)
-                   : xface(std::false_type{}, Pm1, Sm1, P0, S0);
-    } else {
-        Fm = xface(std::false_type{}, Pm1, Sm1, P0, S0);
-    }
+    Fm = split ? xface(std::true_type{}, Pm1, Sm1, P0, S0)
+               : xface(std::false_type{}, Pm1, Sm1, P0, S0);
 
     // The row's faces (the split and IEEE variants): the slope of row r+1, x face r+1/2 and
     // the y faces of row r.
@@ -484,14 +477,10 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
             if constexpr (BATHY) in_h = ld(row_ptr(A.in_h, ro), jcb);
         }
         FaceFlux Fy;
-        if constexpr (kSplitScheme) {
-            if (split)
-                faces(std::true_type{}, P0, P1, P2, S0, S1, Fx, Fy);
-            else
-                faces(std::false_type{}, P0, P1, P2, S0, S1, Fx, Fy);
-        } else {
+        if (split)
+            faces(std::true_type{}, P0, P1, P2, S0, S1, Fx, Fy);
+        else
             faces(std::false_type{}, P0, P1, P2, S0, S1, Fx, Fy);
-        }
         tail(r, P0, Fm, Fx, Fy, u0, in_h);
         finish(Pn, gr);
     };
