@@ -21,8 +21,8 @@
 //   * the positivity floor and non-finite check only with bathymetry (serial_solver.hpp:
 //     190-206); the plain path has neither.
 // Division and sqrt: the split forms of swe_face.h (same bits as `/` and sqrt()) for waves
-// whose loaded cells all pass the modular range guard (mod_split_lanes below), the
-// compiler's IEEE sequences otherwise.  Where the reference's HLL throws (NaN wave speeds,
+// whose cells all pass the modular range guard (mod_in_range / mod_zero_or_in_range below),
+// the compiler's IEEE sequences otherwise.  Where the reference's HLL throws (NaN wave speeds,
 // HLL.hpp:23), the stage raises the non-finite flag instead.
 #include <type_traits>
 

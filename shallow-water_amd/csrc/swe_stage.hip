@@ -57,8 +57,9 @@ struct March {
     double half_g, g, inv_dx, inv_dy, dt;
     double speed;
     int bad;
-    // Wave-uniform: every cell loaded so far by this wave satisfies split_ok, so the faces
-    // may use the split IEEE division/sqrt forms (swe_face.h).  Sticky once false.
+    // Wave-uniform: every cell of the rows finished so far by this wave satisfies split_ok,
+    // so the faces of those rows may use the split IEEE division/sqrt forms (swe_face.h).
+    // Sticky once false.
     bool split;
 
     // Row r as stored (h, hu, hv, b).  Only the loads are issued here: nothing reads the
