@@ -9,15 +9,18 @@ whole grid (three fused stage launches).  Inputs are resident in HBM before timi
 Prints ONE JSON line.  `value` = interior cell-updates per second / 1e9 over the timed
 steps (wall clock around the replayed steps, synchronised).  `roofline` is priced from HIP
 events recorded on the solver's stream around every stage launch in a second timed pass.
-`cpu_baseline` times the reference's own OpenMP kernels (oracle/_ref) on the host on a
-few steps of the same grid; those steps are also compared bitwise with the GPU.
+`cpu_baseline` times the reference's own OpenMP kernels (oracle/_ref) on the host on 10
+steps of the same grid (one thread per available physical core, OMP_PROC_BIND=true,
+OMP_PLACES=cores, in a child process); those steps are also compared bitwise with the GPU.
+Its `c1` record is config 1 in full (256^2, g = 9.81, 1000 steps) on the host and on the
+GPU, with the max relative L-inf difference of the final fields.
 
 Also reported: `batched` (512-instance ensemble vs per-instance launches), `adjoint`
 (1024^2, 2000 steps: adjoint wall time over forward wall time) and `serial_schemes` (the
 reference SerialSolver's reconstruction x Riemann combinations at 4096^2, with the
 reference's own modular code timed on a 1024^2 sample).
 
-    python bench.py [--steps K] [--warmup W] [--n 8192] [--cpu-steps S] [--no-cpu]
+    python bench.py [--steps K] [--warmup W] [--n 8192] [--cpu-steps S] [--no-cpu] [--no-c1]
                     [--no-ensemble] [--no-adjoint] [--no-serial] [--serial-n N]
 """
 from __future__ import annotations
@@ -48,36 +51,70 @@ def mass(h, ng):
 
 
 def cpu_baseline(case, steps, dt):
-    """Time the reference's OpenMP kernels (or the port when the reference build is absent)
-    on `steps` steps of the same grid.  Returns (info dict, final state)."""
-    import oracle
+    """The reference's compiled OpenMP kernels (oracle/_ref; the oracle port when the
+    reference harness is absent) on `steps` fixed-dt steps of the same grid, in a child
+    process with one OpenMP thread per available physical core, OMP_PROC_BIND=true and
+    OMP_PLACES=cores (scripts/leo3e/run_openmp.sh:111-114).  Returns the info dict and the
+    sha256 of the final h, hu, hv (ghosts included) for the bitwise check."""
+    from oracle import cpu_baseline as cb
 
-    gr = oracle.grid(case["nx"], case["ny"], case["ng"], case["dx"], case["dy"], g=case["g"],
-                     bc=case["bc"])
-    h, hu, hv = (case[k].copy() for k in ("h", "hu", "hv"))
-    B = case["B"].copy()
-    oracle.apply_bc(gr, B)
-    if oracle.ref_available():
-        kind, threads = "reference", oracle.ref().ref_num_threads()
-        t0 = time.perf_counter()
-        rc, _ = oracle.ref_run(gr, h, hu, hv, B, steps, dt=dt)
-        el = time.perf_counter() - t0
-    else:
-        kind, threads = "port", oracle.lib().oracle_num_threads()
-        t0 = time.perf_counter()
-        rc, _, _ = oracle.run(gr, h, hu, hv, B, steps, dt=dt)
-        el = time.perf_counter() - t0
-    if rc != 0:
+    host = cb.host_cpus()
+    r = cb.run_child(["--workload", "dam_break", "--n", str(case["nx"]), "--steps", str(steps),
+                      "--dt", float(dt).hex()], host["threads"], timeout=900)
+    if r["rc"] != 0:
         raise RuntimeError("CPU baseline run failed")
     cells = case["nx"] * case["ny"]
     info = {
-        "value": cells * steps / el / 1e9, "unit": "Gcell-updates/s", "cores": threads,
-        "kind": kind,
+        "value": cells * steps / r["seconds"] / 1e9, "unit": "Gcell-updates/s",
+        "cores": r["threads"], "kind": r["kind"],
         "sample": f"{case['nx']}x{case['ny']} dam break, {steps} fixed-dt SSP-RK3 steps "
-                  f"({'oracle/_ref: reference omp_kernels.hpp, OpenMP' if kind == 'reference' else 'oracle port, OpenMP'}"
-                  f", g++ -O3 -ffp-contract=off), {el:.2f} s",
+                  f"({'oracle/_ref: reference omp_kernels.hpp' if r['kind'] == 'reference' else 'oracle port'}"
+                  f", OpenMP, g++ -O3 -ffp-contract=off), {r['seconds']:.2f} s",
+        "threads": r["threads"], "binding": f"OMP_PROC_BIND={r['bind']} OMP_PLACES={r['places']}",
+        "host": host,
     }
-    return info, (h, hu, hv)
+    return info, r["sha256"]
+
+
+def c1_record(threads):
+    """Config 1 as stated: 256^2 Gaussian pulse, g = 9.81, CFL 0.4 dt at t = 0, 1000
+    fixed-dt steps.  The reference's kernels (the g = 9.81 harness build) on the host for
+    the full run, and the GPU on the same dt; Gcell-updates/s of both and the max relative
+    L-inf difference of the final fields (0 when bitwise equal)."""
+    import tempfile
+
+    import swe_hip
+    from oracle import cpu_baseline as cb
+    from swe_hip import workloads as W
+
+    n, steps = 256, 1000
+    with tempfile.TemporaryDirectory() as td:
+        path = os.path.join(td, "c1.npz")
+        r = cb.run_child(["--workload", "gauss_pulse", "--n", str(n), "--steps", str(steps),
+                          "--cfl", "0.4", "--save", path], threads, timeout=900)
+        z = np.load(path)
+        want = [z[k].copy() for k in ("h", "hu", "hv")]
+    dt = float.fromhex(r["dt"])
+    c = W.gauss_pulse(n)
+    with swe_hip.HipSolver.from_case(c) as s:
+        s.upload(c["h"], c["hu"], c["hv"], c["B"])
+        s.step(dt, 2)
+        s.upload(c["h"], c["hu"], c["hv"], c["B"])
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(dt, steps)
+        el = time.perf_counter() - t0
+        got = s.download()
+    linf = 0.0
+    for a, b in zip(got, want):
+        scale = float(np.max(np.abs(b)))
+        linf = max(linf, float(np.max(np.abs(a - b))) / (scale if scale > 0 else 1.0))
+    bitwise = all(np.array_equal(a.view(np.uint64), b.view(np.uint64)) for a, b in zip(got, want))
+    return {"workload": "gauss_pulse_256x256_g9.81_cfl0.4_1000_steps", "dt": dt,
+            "cpu_kind": r["kind"], "cpu_threads": r["threads"],
+            "cpu_gcell_per_s": round(n * n * steps / r["seconds"] / 1e9, 5),
+            "gpu_gcell_per_s": round(n * n * steps / el / 1e9, 4),
+            "max_rel_linf": linf, "bitwise": bool(bitwise)}
 
 
 def ensemble_bench(steps, base_steps, check_every=32):
@@ -98,7 +135,9 @@ def ensemble_bench(steps, base_steps, check_every=32):
         b.upload(i, c["h"], c["hu"], c["hv"], c["B"])
     b.stable_dt(0.4)
     tile_rows, waves = b.autotune()
-    b.step_adaptive(0.4, 2)   # warmup: graph capture
+    # warmup with the timed step count: the dt history and the captured step graph are
+    # sized and built before the timed region
+    b.step_adaptive(0.4, steps)
     for i, c in enumerate(cases):
         b.upload(i, c["h"], c["hu"], c["hv"], c["B"])
     t0 = time.perf_counter()
@@ -113,7 +152,7 @@ def ensemble_bench(steps, base_steps, check_every=32):
     for c in cases:
         s = swe_hip.HipSolver.from_case(c)
         s.upload(c["h"], c["hu"], c["hv"], c["B"])
-        s.step_adaptive(0.4, 2)   # warmup / graph capture
+        s.step_adaptive(0.4, base_steps)   # warmup: history sized, step graph captured
         s.upload(c["h"], c["hu"], c["hv"], c["B"])
         solvers.append(s)
     t0 = time.perf_counter()
@@ -128,7 +167,7 @@ def ensemble_bench(steps, base_steps, check_every=32):
     for s, c in zip(solvers, cases):
         s.set_graph(False)
         s.upload(c["h"], c["hu"], c["hv"], c["B"])
-        s.step_adaptive(0.4, 1)
+        s.step_adaptive(0.4, base_steps)   # warmup with the timed step count
         s.upload(c["h"], c["hu"], c["hv"], c["B"])
     solvers[-1].sync()
     t0 = time.perf_counter()
@@ -188,10 +227,16 @@ def adjoint_bench(n=1024, steps=2000):
     res = {"workload": f"adjoint_{n}x{n}_{steps}_steps (16-bump bathymetry, surface bump, "
                        "background undulation + flow), gauge-misfit loss at 16 cells",
            "forward_s": round(t_fwd, 4)}
-    for label, nsnap in (("auto", 0), ("binomial_64", 64)):
+    arr = (n + 4) * (n + 4) * 8
+    store_all = 9.0 * arr * steps   # every step with its two stage states
+    runs = [("auto", 0, None), ("records_25pct", 0, 0.25 * store_all),
+            ("records_10pct", 0, 0.10 * store_all)]
+    runs += [(f"binomial_{k}", k, None) for k in (64, 128, 256, 512, steps)]
+    ref_grad = None
+    for label, nsnap, budget in runs:
         with swe_hip.HipSolver.from_case(case) as s:
             s.upload(case["h"], case["hu"], case["hv"], case["B"])
-            with swe_hip.HipAdjoint(s, steps, nsnap) as adj:
+            with swe_hip.HipAdjoint(s, steps, nsnap, budget_bytes=budget) as adj:
                 t0 = time.perf_counter()
                 adj.forward(dt=dt)
                 hN = s.download()[0]
@@ -201,14 +246,27 @@ def adjoint_bench(n=1024, steps=2000):
                 g = adj.backward(lam, z, z)
                 t_adj = time.perf_counter() - t0
                 info = adj.info()
+                plan = adj.plan()
         res[label] = {"adjoint_s": round(t_adj, 4), "ratio": round(t_adj / t_fwd, 3),
                       "nsnap": info["nsnap"], "stage_records": info["records"],
+                      "record_steps": plan["record_steps"], "ckpt_every": plan["ckpt_every"],
+                      "store_gb": round(plan["store_bytes"] / 1e9, 3),
+                      "store_frac_of_store_all": round(plan["store_bytes"] / store_all, 4),
+                      "recomputed_steps": round(plan["recomputed_steps"], 1),
                       "forward_ms": round(info["forward_ms"], 1),
                       "backward_ms": round(info["backward_ms"], 1),
                       "grad_finite": bool(np.isfinite(g[0]).all() and np.isfinite(g[3]).all())}
+        if ref_grad is None:
+            ref_grad = g
+        res[label]["grad_bitwise_vs_auto"] = all(
+            np.array_equal(a.view(np.uint64), b.view(np.uint64)) for a, b in zip(g, ref_grad))
         log(f"[bench] adjoint {label}: {t_adj:.3f} s vs forward {t_fwd:.3f} s "
-            f"-> {t_adj / t_fwd:.2f}x ({info})")
+            f"-> {t_adj / t_fwd:.2f}x ({info}, {plan})")
     res["ratio"] = res["auto"]["ratio"]
+    # the least stored memory among the runs that stay within 4x the forward time
+    ok = [(v["store_frac_of_store_all"], k) for k, v in res.items()
+          if isinstance(v, dict) and v.get("ratio", 99) <= 4.0]
+    res["least_memory_within_4x"] = min(ok)[1] if ok else None
     return res
 
 
@@ -285,7 +343,8 @@ def main():
     ap.add_argument("--steps", type=int, default=1000)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--n", type=int, default=8192)
-    ap.add_argument("--cpu-steps", type=int, default=2)
+    ap.add_argument("--cpu-steps", type=int, default=10)
+    ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-rows", type=int, default=0)
     ap.add_argument("--event-steps", type=int, default=20)
@@ -382,22 +441,23 @@ def main():
     s.close()
 
     if not args.no_cpu:
+        from oracle import cpu_baseline as cb
+
         log(f"[bench] CPU baseline: {args.cpu_steps} steps")
-        try:
-            info, ref_state = cpu_baseline(case, args.cpu_steps, dt)
-            out["cpu_baseline"] = info
-            # same steps on the GPU, compared bitwise
-            with swe_hip.HipSolver.from_case(case) as s2:
-                s2.upload(case["h"], case["hu"], case["hv"], case["B"])
-                s2.step(dt, args.cpu_steps)
-                got = s2.download()
-            same = all(np.array_equal(a.view(np.uint64), b.view(np.uint64))
-                       for a, b in zip(got, ref_state))
-            out["cpu_parity_bitwise"] = bool(same)
-            log(f"[bench] cpu {info['value']:.4f} Gcell/s ({info['cores']} threads), bitwise={same}")
-        except MemoryError as e:
-            out["cpu_baseline"] = None
-            log(f"[bench] CPU baseline skipped: {e}")
+        info, ref_sha = cpu_baseline(case, args.cpu_steps, dt)
+        # same steps on the GPU, compared bitwise (sha256 of the full arrays)
+        with swe_hip.HipSolver.from_case(case) as s2:
+            s2.upload(case["h"], case["hu"], case["hv"], case["B"])
+            s2.step(dt, args.cpu_steps)
+            got = s2.download()
+        same = cb.digest(*got) == ref_sha
+        out["cpu_parity_bitwise"] = bool(same)
+        log(f"[bench] cpu {info['value']:.4f} Gcell/s ({info['cores']} threads, "
+            f"{info['binding']}, host {info['host']}), bitwise={same}")
+        if not args.no_c1:
+            info["c1"] = c1_record(info["threads"])
+            log(f"[bench] C1: {info['c1']}")
+        out["cpu_baseline"] = info
     if not args.no_ensemble:
         out["batched"] = ensemble_bench(args.ensemble_steps, args.ensemble_base_steps)
     if not args.no_adjoint:

@@ -88,7 +88,8 @@ int swe_hip_abi_version(void);
 /* Status code -> static string. */
 const char *swe_hip_error_string(int code);
 
-/* Detailed message of the last failure on ctx ("" if none). */
+/* Detailed message of the last failure on ctx ("" if none).  With ctx == NULL: the message
+ * of the calling thread's last failed swe_hip_create (which leaves no context behind). */
 const char *swe_hip_last_error(const swe_hip_ctx *ctx);
 
 /* Number of visible HIP devices (0 when there is no GPU). */
@@ -136,8 +137,9 @@ int swe_hip_step_adaptive(swe_hip_ctx *ctx, double cfl, int64_t nsteps, double *
  * host, interior written, ghosts 0 (as the reference's zero-initialized vectors). */
 int swe_hip_rhs(swe_hip_ctx *ctx, double *rhs_h, double *rhs_hu, double *rhs_hv);
 
-/* Launch shape of the fused stage kernel: interior rows marched per wave (>= 1).
- * Default picked by swe_hip_create from the grid size. */
+/* Launch shape of the fused stage kernel: interior rows marched per wave (>= 1; values
+ * above nx are stored as nx, one row segment).  Default picked by swe_hip_create from the
+ * grid size. */
 int swe_hip_set_tile_rows(swe_hip_ctx *ctx, int tile_rows);
 int swe_hip_get_tile_rows(const swe_hip_ctx *ctx);
 
@@ -235,11 +237,34 @@ typedef enum {
     SWE_DT_GIVEN = 2     /* a caller-given dt per step */
 } swe_dt_mode;
 
-/* nsnap = number of stored states, U^0 included (>= 1).  -1 stores every step together with
- * its two stage states (the backward then recomputes nothing; 9 arrays per step).  0 picks
- * -1 when that fits in 60% of the free device memory, else as many states as half of it
- * holds (at most nsteps). */
+/* nsnap = number of stored states, U^0 included (>= 1): binomial checkpoints, each reversed
+ * step recomputing its two stage states.  -1 stores every step together with its two stage
+ * states (the backward then recomputes nothing; 9 arrays per step).  0 picks the storage
+ * plan of swe_hip_adj_create_budget for 60% of the free device memory. */
 int swe_hip_adj_create(swe_hip_ctx *ctx, int64_t nsteps, int32_t nsnap, swe_hip_adj **out);
+
+/* Adjoint whose stored states take at most max_bytes of device memory.  Plan: the last K
+ * steps keep their stage records (U^p, U1, U2: 9 arrays) from the forward pass, and the
+ * steps before them a checkpoint every m steps (3 arrays); the backward recomputes each of
+ * those steps once, segment by segment into the record buffer.  K (the most that fits) and
+ * m are chosen for the budget; K = nsteps when every record fits.  Below the smallest such
+ * plan: binomial checkpoints in as many states as fit.  SWE_ERR_OOM when not one state fits.
+ * No counterpart in the reference. */
+int swe_hip_adj_create_budget(swe_hip_ctx *ctx, int64_t nsteps, double max_bytes,
+                              swe_hip_adj **out);
+
+/* The plan swe_hip_adj_create_budget would pick for nsteps steps of a grid with n_total
+ * cells per array, computed on the host (no device needed): recorded steps K (0: binomial
+ * fallback), checkpoint interval m (0 for binomial) and stored states.  SWE_ERR_OOM when not
+ * one state fits. */
+int swe_hip_adj_plan_budget(int64_t nsteps, int64_t n_total, double max_bytes,
+                            int64_t *record_steps, int64_t *ckpt_every, int64_t *stored_states);
+
+/* The storage plan in use: recorded steps K (0 for binomial checkpoints), checkpoint
+ * interval m, stored states (records + checkpoints, or binomial slots), their bytes, and
+ * the forward work the backward pass recomputes, in steps.  Any pointer may be NULL. */
+int swe_hip_adj_plan(const swe_hip_adj *adj, int64_t *record_steps, int64_t *ckpt_every,
+                     int64_t *stored_states, double *store_bytes, double *recomputed_steps);
 void swe_hip_adj_destroy(swe_hip_adj *adj);
 const char *swe_hip_adj_last_error(const swe_hip_adj *adj);
 
@@ -274,8 +299,9 @@ int swe_hip_adj_set_launch(swe_hip_adj *adj, int block_waves, int rows, int y_ro
 int swe_hip_adj_schedule(int64_t nsteps, int32_t nsnap, int64_t *advances, int32_t *max_reps,
                          int64_t *actions);
 
-/* The number of stored states in use, whether stage states are stored too (1/0), and the
- * wall-clock milliseconds of the last forward and backward calls.  Any pointer may be NULL. */
+/* The number of stored states in use, whether stage records are stored (1/0: the records
+ * plan / binomial checkpoints), and the wall-clock milliseconds of the last forward and
+ * backward calls.  Any pointer may be NULL. */
 int swe_hip_adj_info(const swe_hip_adj *adj, int32_t *nsnap, int32_t *records,
                      double *forward_ms, double *backward_ms);
 

@@ -15,7 +15,8 @@ Two libraries:
 * ``_ref/libswe_ref.so`` -- the reference's own header-only kernels compiled where they
   lie (oracle/ref_driver.cpp + ``make -C oracle ref``).  Present only where the
   reference tree was available at build time (the .so itself travels with the tree).
-  g is fixed at the reference's 9.80665.
+  g is fixed at the reference's 9.80665; ``_ref/libswe_ref_g981.so`` is the same build
+  with g = 9.81 (config 1 as stated) through the shadow ``g981/include/constants.hpp``.
 * ``_ref/libswe_ref_modular.so`` -- likewise, the reference's modular serial solver parts
   (reconstructions, Riemann solvers, flux assemblies, SSP-RK3; oracle/ref_modular.cpp).
 
@@ -38,6 +39,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 ORACLE_SO = os.path.join(HERE, "liboracle.so")
 TANGENT_SO = os.path.join(HERE, "libswe_tangent.so")
 REF_SO = os.path.join(HERE, "_ref", "libswe_ref.so")
+REF_G981_SO = os.path.join(HERE, "_ref", "libswe_ref_g981.so")
 REF_MOD_SO = os.path.join(HERE, "_ref", "libswe_ref_modular.so")
 
 # modular scheme selectors (oracle/swe_modular.c; reconstruction / Riemann solver)
@@ -124,29 +126,52 @@ def ref():
     """The reference harness, or None when it was not built."""
     global _ref
     if _ref is None and ref_available():
-        R = C.CDLL(REF_SO)
-        R.ref_g.restype = C.c_double
-        R.ref_num_threads.restype = C.c_int
-        for nm in ("ref_minmod", "hpc_minmod"):
-            getattr(R, nm).restype = C.c_double
-            getattr(R, nm).argtypes = [C.c_double, C.c_double]
-        for nm in ("ref_hll_x", "ref_hll_y"):
-            getattr(R, nm).restype = None
-            getattr(R, nm).argtypes = [C.c_double] * 6 + [_dp]
-        grid = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]
-        for nm in ("ref_faces_and_rhs", "hpc_faces_and_rhs"):
-            getattr(R, nm).restype = None
-            getattr(R, nm).argtypes = grid + [_dp] * 4 + [C.POINTER(_dp), C.POINTER(_dp)]
-        R.ref_stable_dt.restype = C.c_double
-        R.ref_stable_dt.argtypes = grid + [_dp] * 3 + [C.c_double]
-        R.ref_enforce_positivity.restype = C.c_int
-        R.ref_enforce_positivity.argtypes = grid + [_dp] * 3
-        R.ref_reflect.restype = None
-        R.ref_reflect.argtypes = grid + [_dp] * 3
-        R.ref_run.restype = C.c_int
-        R.ref_run.argtypes = grid + [_dp] * 4 + [C.c_int, C.c_double, C.c_long, _dp]
-        _ref = R
+        _ref = _load_ref(REF_SO)
     return _ref
+
+
+_ref_g981 = None
+
+
+def ref_for_g(g: float):
+    """The reference harness compiled with gravity g (9.80665, the reference's own value,
+    or 9.81 through the shadow constants header), or None when that build is absent."""
+    global _ref_g981
+    if g == G_REF:
+        return ref()
+    if g == 9.81:
+        if _ref_g981 is None and os.path.exists(REF_G981_SO):
+            _ref_g981 = _load_ref(REF_G981_SO)
+        return _ref_g981
+    return None
+
+
+def _load_ref(path):
+    R = C.CDLL(path)
+    R.ref_g.restype = C.c_double
+    R.ref_num_threads.restype = C.c_int
+    for nm in ("ref_minmod", "hpc_minmod"):
+        getattr(R, nm).restype = C.c_double
+        getattr(R, nm).argtypes = [C.c_double, C.c_double]
+    for nm in ("ref_hll_x", "ref_hll_y"):
+        getattr(R, nm).restype = None
+        getattr(R, nm).argtypes = [C.c_double] * 6 + [_dp]
+    grid = [C.c_int, C.c_int, C.c_int, C.c_double, C.c_double]
+    for nm in ("ref_faces_and_rhs", "hpc_faces_and_rhs"):
+        getattr(R, nm).restype = None
+        getattr(R, nm).argtypes = grid + [_dp] * 4 + [C.POINTER(_dp), C.POINTER(_dp)]
+    R.ref_stable_dt.restype = C.c_double
+    R.ref_stable_dt.argtypes = grid + [_dp] * 3 + [C.c_double]
+    R.ref_enforce_positivity.restype = C.c_int
+    R.ref_enforce_positivity.argtypes = grid + [_dp] * 3
+    R.ref_reflect.restype = None
+    R.ref_reflect.argtypes = grid + [_dp] * 3
+    R.ref_run.restype = C.c_int
+    R.ref_run.argtypes = grid + [_dp] * 4 + [C.c_int, C.c_double, C.c_long, _dp]
+    R.ref_initial_state.restype = C.c_int
+    R.ref_initial_state.argtypes = ([C.c_int] * 3 + [C.c_double] * 2 + [C.c_int, _dp, C.c_int,
+                                                                         _dp] + [_dp] * 4)
+    return R
 
 
 _refmod = None
@@ -312,13 +337,39 @@ def ref_faces_and_rhs(gr: OracleGrid, h, hu, hv, B, hpc=False):
     return f, r
 
 
-def ref_run(gr: OracleGrid, h, hu, hv, B, nsteps, dt=None, cfl=None):
+def ref_run(gr: OracleGrid, h, hu, hv, B, nsteps, dt=None, cfl=None, lib=None):
+    """The reference kernels' rk3_step loop (lib: a harness from ref_for_g; default the
+    reference's own g)."""
     assert (dt is None) != (cfl is None)
+    R = lib if lib is not None else ref()
     hist = np.zeros(max(nsteps, 1))
     mode, val = (0, dt) if dt is not None else (1, cfl)
-    rc = ref().ref_run(gr.nx, gr.ny, gr.ng, gr.dx, gr.dy, _p(h), _p(hu), _p(hv), _p(B), mode, val,
-                       nsteps, _p(hist))
+    rc = R.ref_run(gr.nx, gr.ny, gr.ng, gr.dx, gr.dy, _p(h), _p(hu), _p(hv), _p(B), mode, val,
+                   nsteps, _p(hist))
     return rc, hist[:nsteps]
+
+
+# reference enum values (configs/config.hpp:11-13)
+IC_TYPES = {"GaussInitial": 0, "StillWater": 1, "DamBreak": 2, "DamBreakRadial": 3}
+BATHY_TYPES = {"None": 0, "Flat": 1, "GaussHill": 2}
+IC_KEYS = ("peak_height", "x0", "y0", "sigma_x", "sigma_y", "h0", "dam_height", "dam_x",
+           "dam_x0", "dam_y0", "dam_radius")
+BATHY_KEYS = ("b0", "bathy_x0", "bathy_y0", "bathy_sigma_x", "bathy_sigma_y",
+              "bathy_peak_height")
+
+
+def ref_initial_state(nx, ny, ng, lx, ly, ic_type: str, ic: dict, bathy_type: str, bt: dict):
+    """The reference's own setup code (apply_initial_condition / apply_bathymetry) and the
+    solver constructor's ghost fills (omp_solver.hpp:118-127): full arrays h, hu, hv, B."""
+    shape = (nx + 2 * ng, ny + 2 * ng)
+    out = [np.zeros(shape) for _ in range(4)]
+    icv = np.array([float(ic.get(k, 0.0)) for k in IC_KEYS])
+    btv = np.array([float(bt.get(k, 0.0)) for k in BATHY_KEYS])
+    rc = ref().ref_initial_state(nx, ny, ng, lx, ly, IC_TYPES[ic_type], _p(icv),
+                                 BATHY_TYPES[bathy_type], _p(btv), *(_p(a) for a in out))
+    if rc != 0:
+        raise ValueError("reference setup code threw")
+    return out
 
 
 _tan = None

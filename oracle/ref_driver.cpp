@@ -10,8 +10,13 @@
 // (include/io/netCDF_writer.hpp:8): the rk3_step sequence of omp_solver.hpp:250-338 and
 // the private ghost fill of omp_solver.hpp:340-432, restated so the reference kernels
 // can be driven over raw arrays.  g is the reference's compile-time constant 9.80665.
+// ref_initial_state runs the reference's own setup code (apply_bathymetry.hpp,
+// apply_initial_condition.hpp) the way the solver constructor does
+// (omp_solver.hpp:118-127), for the host application's initial-state parity.
 #include "include/backend/openmp/omp_kernels.hpp"
 #include "include/backend/serial/HPC/HPC_kernels.hpp"
+#include "include/bathymetry/apply_bathymetry.hpp"
+#include "include/initial_condition/apply_initial_condition.hpp"
 
 #include <cstdint>
 #include <cstring>
@@ -209,6 +214,62 @@ int ref_run(int nx, int ny, int ng, double dx, double dy, double *h, double *hu,
         }
     } catch (const std::runtime_error &) {
         return 1;
+    } catch (...) {
+        return -1;
+    }
+    return 0;
+}
+
+// The solver constructor's initial arrays (omp_solver.hpp:118-127): the reference's
+// apply_bathymetry then the scalar ghost fill of B, apply_initial_condition then the ghost
+// fill of (h, hu, hv).  ic = {peak_height, x0, y0, sigma_x, sigma_y, h0, dam_height, dam_x,
+// dam_x0, dam_y0, dam_radius}; bt = {b0, bathy_x0, bathy_y0, bathy_sigma_x, bathy_sigma_y,
+// bathy_peak_height}; the types are the reference enums' values.  Returns 0, or -1 when the
+// reference throws (unsupported type).
+int ref_initial_state(int nx, int ny, int ng, double lx, double ly, int ic_type,
+                      const double *ic, int bathy_type, const double *bt, double *h,
+                      double *hu, double *hv, double *B) {
+    try {
+        SimulationConfig cfg{};
+        cfg.mesh.Lx = lx;
+        cfg.mesh.Ly = ly;
+        cfg.mesh.nG = ng;
+        cfg.mesh.Nx = nx;
+        cfg.mesh.Ny = ny;
+        auto &c = cfg.initial_condition;
+        c.type = static_cast<InitialConditionType>(ic_type);
+        c.peak_height = ic[0];
+        c.x0 = ic[1];
+        c.y0 = ic[2];
+        c.sigma_x = ic[3];
+        c.sigma_y = ic[4];
+        c.h0 = ic[5];
+        c.dam_height = ic[6];
+        c.dam_x = ic[7];
+        c.dam_x0 = ic[8];
+        c.dam_y0 = ic[9];
+        c.dam_radius = ic[10];
+        auto &b = cfg.bathymetry;
+        b.type = static_cast<BathymetryType>(bathy_type);
+        b.b0 = bt[0];
+        b.bathy_x0 = bt[1];
+        b.bathy_y0 = bt[2];
+        b.bathy_sigma_x = bt[3];
+        b.bathy_sigma_y = bt[4];
+        b.bathy_peak_height = bt[5];
+        const Grid grid(nx, ny, lx, ly, ng);
+        const auto v = make_view(nx, ny, ng, lx / nx, ly / ny);
+        const std::size_t n = static_cast<std::size_t>(grid.Nx_total()) * grid.Ny_total();
+        Array2D Bt(grid.Nx_total(), grid.Ny_total());
+        apply_bathymetry(cfg, grid, Bt);
+        std::memcpy(B, Bt.data(), n * sizeof(double));
+        reflect(v, B, nullptr, nullptr);
+        State U(grid);
+        apply_initial_condition(cfg, grid, U);
+        std::memcpy(h, U.h().data(), n * sizeof(double));
+        std::memcpy(hu, U.hu().data(), n * sizeof(double));
+        std::memcpy(hv, U.hv().data(), n * sizeof(double));
+        reflect(v, h, hu, hv);
     } catch (...) {
         return -1;
     }

@@ -788,14 +788,17 @@ void adjoint_stage_kernel(const GridDev G, const AdjArgs A) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wv = __builtin_amdgcn_readfirstlane((int)(threadIdx.x / kWave));
     const Lanes<BC> w(G, A, (int)blockIdx.x, lane);
-    const int ng = G.ng, nx = G.nx, T = A.tile_rows;
-    const int R0 = ng + (int)blockIdx.y * (W * T);
-    const int Rend = (R0 + W * T < ng + nx) ? R0 + W * T : ng + nx;
+    const int ng = G.ng, nx = G.nx;
+    // row blocks of (nearly) equal height: block y takes rows [R0, Rend), at most
+    // A.block_rows of them (the LDS layout)
+    const int nb = (int)gridDim.y;
+    const int R0 = ng + (int)(((long long)blockIdx.y * nx) / nb);
+    const int Rend = ng + (int)(((long long)(blockIdx.y + 1) * nx) / nb);
     const int F0 = (R0 - 2 > ng - 1) ? R0 - 2 : ng - 1;
     const int F1 = (Rend < ng + nx - 1) ? Rend : ng + nx - 1;
     const int NF = F1 - F0 + 1;
     unsigned char *lds_sel =
-        reinterpret_cast<unsigned char *>(lds_faces + (long)(W * T + 3) * 8 * kWave);
+        reinterpret_cast<unsigned char *>(lds_faces + (long)(A.block_rows + 3) * 8 * kWave);
     constexpr double cv = STAGE == 1 ? 1.0 : (STAGE == 2 ? 0.25 : 2.0 / 3.0);
 
     // ---- phase 1: x faces fa .. fb-1 of this wave
@@ -873,9 +876,9 @@ void adjoint_stage_kernel(const GridDev G, const AdjArgs A) {
     }
     __syncthreads();
 
-    // ---- phase 2: rows ra .. rb-1 of this wave
-    const int ra = R0 + wv * T;
-    const int rb = (ra + T < Rend) ? ra + T : Rend;
+    // ---- phase 2: rows ra .. rb-1 of this wave (an equal share of the block's rows)
+    const int ra = R0 + (wv * (Rend - R0)) / W;
+    const int rb = R0 + ((wv + 1) * (Rend - R0)) / W;
     if (ra >= rb) return;
     const double ky = G.inv_dy * (cv * *A.dt);
     const LdsGather X{lds_faces, lds_sel, F0, lane};
@@ -955,15 +958,65 @@ hipError_t launch_adj_one(const GridDev &G, const AdjArgs &A, hipStream_t stream
     return hipGetLastError();
 }
 
+// Row blocks of the fused kernel.  A block holds at most cap = W * tile_rows rows (its LDS
+// layout); with C blocks resident at once the launch takes ceil(nstrips * nb / C) rounds of
+// a block's time, which grows with its rows plus the 3 extra x faces it recomputes.  The
+// count nb (>= ceil(nx / cap)) minimising rounds x (rows + 3) avoids a nearly empty last
+// round (1024^2 at cap 12: 86 blocks per strip in 4 rounds, the last one 2% full).
+int adj_row_blocks(int nx, int nstrips, int cap, long long resident) {
+    const int lo = (nx + cap - 1) / cap;
+    int best = lo;
+    long long best_cost = -1;
+    for (int nb = lo; nb <= nx && nb <= 4 * lo + 64; ++nb) {
+        const long long rounds = ((long long)nstrips * nb + resident - 1) / resident;
+        const long long cost = rounds * ((nx + nb - 1) / nb + 3);
+        if (best_cost < 0 || cost < best_cost) {
+            best_cost = cost;
+            best = nb;
+        }
+    }
+    return best;
+}
+
+// Blocks of one fused-kernel instance resident on the device at once (CUs x occupancy),
+// cached per kernel and LDS size.
+long long adj_resident_blocks(const void *fn, int block_threads, size_t lds) {
+    struct Key {
+        const void *fn;
+        size_t lds;
+        int dev;
+        long long n;
+    };
+    static thread_local std::vector<Key> cache;
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) return 256;
+    for (const Key &k : cache)
+        if (k.fn == fn && k.lds == lds && k.dev == dev) return k.n;
+    int cus = 0, per_cu = 0;
+    if (hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) != hipSuccess ||
+        hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, fn, block_threads, lds) !=
+            hipSuccess ||
+        cus < 1 || per_cu < 1)
+        return 256;
+    const long long n = (long long)cus * per_cu;
+    cache.push_back(Key{fn, lds, dev, n});
+    return n;
+}
+
 template <int STAGE, int BC, int W>
-hipError_t launch_fused_w(const GridDev &G, const AdjArgs &A, hipStream_t stream) {
-    const size_t lds = adj_lds_bytes(W, A.tile_rows);
+hipError_t launch_fused_w(const GridDev &G, const AdjArgs &A0, hipStream_t stream) {
+    AdjArgs A = A0;
+    const int cap = W * A.tile_rows;
     const void *fn = reinterpret_cast<const void *>(&adjoint_stage_kernel<STAGE, BC, W>);
-    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+    const size_t lds_cap = adj_lds_bytes(W, A.tile_rows);
+    hipError_t e = hipFuncSetAttribute(fn, hipFuncAttributeMaxDynamicSharedMemorySize,
+                                       (int)lds_cap);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL((adjoint_stage_kernel<STAGE, BC, W>),
-                       dim3((unsigned)A.nstrips,
-                            (unsigned)((G.nx + W * A.tile_rows - 1) / (W * A.tile_rows))),
+    const long long resident = adj_resident_blocks(fn, kWave * W, lds_cap);
+    const int nb = adj_row_blocks(G.nx, A.nstrips, cap, resident);
+    A.block_rows = (G.nx + nb - 1) / nb;  // <= cap
+    const size_t lds = adj_lds_rows_bytes(A.block_rows);
+    hipLaunchKernelGGL((adjoint_stage_kernel<STAGE, BC, W>), dim3((unsigned)A.nstrips, (unsigned)nb),
                        dim3(kWave * W), lds, stream, G, A);
     return hipGetLastError();
 }
@@ -993,9 +1046,13 @@ hipError_t launch_adj_bc(int stage, const GridDev &G, const AdjArgs &A, hipStrea
 
 int adj_strips_for(int ny) { return (ny + kAdjCols - 1) / kAdjCols; }
 
-size_t adj_lds_bytes(int block_waves, int tile_rows) {
-    const size_t faces = (size_t)block_waves * tile_rows + 3;
+size_t adj_lds_rows_bytes(int block_rows) {
+    const size_t faces = (size_t)block_rows + 3;
     return faces * 8 * kWave * sizeof(double) + (faces + 1) * kWave;
+}
+
+size_t adj_lds_bytes(int block_waves, int tile_rows) {
+    return adj_lds_rows_bytes(block_waves * tile_rows);
 }
 
 bool adj_launch_ok(int block_waves, int tile_rows) {

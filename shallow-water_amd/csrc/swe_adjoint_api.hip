@@ -15,6 +15,16 @@
 // forward stage kernels, then run the adjoint stage kernels 3, 2, 1.  The floor masks of
 // the stage outputs travel between the adjoint kernels (each writes the mask of its own
 // input), so no stage output is ever recomputed for them.
+//
+// Stage records.  The other storage plan keeps "records" -- U^p, U1 and U2 of a step, the
+// three inputs its adjoint stages read -- for the last K steps straight from the forward
+// pass, and a checkpoint of U^p every m steps before them.  The backward reverses the
+// recorded steps as they are, then, segment by segment from the last, recomputes the m
+// steps of a segment from its checkpoint into the record buffer (stage 3 of the segment's
+// last step is not needed) and reverses them.  Every step before the recorded tail is
+// recomputed exactly once.  K = nsteps is "store every step".  plan_records picks m and K
+// for a memory budget: the largest K (fewest recomputed steps) whose records, max(K, m)
+// x 9 arrays, and checkpoints, ceil((nsteps - K) / m) x 3 arrays, fit.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -90,6 +100,39 @@ std::vector<Action> make_schedule(int64_t nsteps, int32_t nsnap) {
     return out;
 }
 
+// Records plan (see the header): K recorded steps, a checkpoint every m steps before them.
+struct RecordPlan {
+    int64_t K = 0, m = 1, nck = 0;
+    int64_t arrays() const { return 9 * std::max(K, m) + 3 * nck; }
+};
+
+int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
+
+// The plan with the most recorded steps whose storage fits `arrays_budget` arrays (ties:
+// the smaller storage); K = 0 when none fits.
+RecordPlan plan_records(int64_t nsteps, int64_t arrays_budget) {
+    RecordPlan best;
+    if (9 * nsteps <= arrays_budget) {
+        best.K = nsteps;
+        return best;
+    }
+    for (int64_t m = 1; m <= nsteps; ++m) {
+        auto need = [&](int64_t K) { return 9 * std::max(K, m) + 3 * ceil_div(nsteps - K, m); };
+        if (need(std::min(m, nsteps - 1)) > arrays_budget) continue;
+        int64_t lo = std::min(m, nsteps - 1), hi = nsteps - 1;  // need() rises with K
+        while (lo < hi) {
+            const int64_t mid = lo + (hi - lo + 1) / 2;
+            if (need(mid) <= arrays_budget)
+                lo = mid;
+            else
+                hi = mid - 1;
+        }
+        RecordPlan cand{lo, m, ceil_div(nsteps - lo, m)};
+        if (cand.K > best.K || (cand.K == best.K && cand.arrays() < best.arrays())) best = cand;
+    }
+    return best;
+}
+
 }  // namespace
 
 struct swe_hip_adj {
@@ -97,8 +140,13 @@ struct swe_hip_adj {
     int64_t nsteps = 0;
     int32_t nsnap = 0;
     size_t n_total = 0;
-    double *slots = nullptr;  // nsnap states of 3 arrays (9 with stage records)
-    bool records = false;     // slot p also holds U1, U2 of step p (store-all only)
+    double *slots = nullptr;  // binomial: nsnap states of 3 arrays; records plan: the
+                              // ckpt states (3 arrays each)
+    bool records = false;     // records plan (stage records; binomial otherwise)
+    int64_t rec_steps = 0;    // records plan: K recorded steps (the last K of the run)
+    int64_t ckpt_every = 1;   // records plan: checkpoint interval m before them
+    int64_t nck = 0;          // records plan: ceil((nsteps - K) / m) checkpoints
+    double *rec = nullptr;    // records plan: max(K, m) records of 9 arrays (U^p, U1, U2)
     int per_slot = 3;
     double *W[3] = {nullptr, nullptr, nullptr};
     double *lam[4][3] = {};   // step adjoint, dL/dU2, dL/dU1, U^n accumulator
@@ -170,21 +218,21 @@ swe::StageArgs fwd_args(swe_hip_adj *a, const State &in, const State &u0, const 
     return s;
 }
 
-// Where stages 1 and 2 of step p go: the step's record (stage records: slot p holds U^p,
-// U1, U2) or the context's stage buffers.
-State stage_state(swe_hip_adj *a, int64_t p, int which) {
-    swe_hip_ctx *c = a->ctx;
-    if (a->records)
-        return State{{slot_ptr(a, (int)p, 3 * which), slot_ptr(a, (int)p, 3 * which + 1),
-                      slot_ptr(a, (int)p, 3 * which + 2)}};
-    double *const *b = which == 1 ? c->U1 : c->U2;
+// The context's stage buffers (stage states of a step that is not recorded).
+State scratch_stage(swe_hip_adj *a, int which) {
+    double *const *b = which == 1 ? a->ctx->U1 : a->ctx->U2;
     return State{{b[0], b[1], b[2]}};
 }
 
-// Stages 1 and 2 of step p from `base`.
-int recompute_u1_u2(swe_hip_adj *a, const State &base, int64_t p) {
+// Record r of the records plan: which = 0 (U^p), 1 (U1), 2 (U2).
+State rec_state(swe_hip_adj *a, int64_t r, int which) {
+    double *base = a->rec + ((size_t)r * 9 + 3 * (size_t)which) * a->n_total;
+    return State{{base, base + a->n_total, base + 2 * a->n_total}};
+}
+
+// Stages 1 and 2 of step p from `base` into u1, u2.
+int stages_12(swe_hip_adj *a, const State &base, const State &u1, const State &u2, int64_t p) {
     swe_hip_ctx *c = a->ctx;
-    const State u1 = stage_state(a, p, 1), u2 = stage_state(a, p, 2);
     ADJ_CHECK(a, swe::launch_stage(1, c->bc, false, false, c->G,
                                    fwd_args(a, base, base, u1, a->dts + p), c->stream, nullptr));
     ADJ_CHECK(a, swe::launch_stage(2, c->bc, false, false, c->G,
@@ -192,9 +240,10 @@ int recompute_u1_u2(swe_hip_adj *a, const State &base, int64_t p) {
     return SWE_OK;
 }
 
-// One forward step p: in -> out (out may alias in).  With cfl > 0, dt_p is first computed
-// from `in` (compute_stable_dt) and recorded in dts[p].
-int forward_step(swe_hip_adj *a, const State &in, const State &out, int64_t p, double cfl) {
+// One forward step p: in -> out (out may alias in) through the stage states u1, u2.  With
+// cfl > 0, dt_p is first computed from `in` (compute_stable_dt) and recorded in dts[p].
+int forward_step_via(swe_hip_adj *a, const State &in, const State &u1, const State &u2,
+                     const State &out, int64_t p, double cfl) {
     swe_hip_ctx *c = a->ctx;
     if (cfl > 0.0) {
         ADJ_CHECK(a, swe::launch_speed_reduce(c->G, in.p[0], in.p[1], in.p[2], a->partial, 1024,
@@ -202,12 +251,15 @@ int forward_step(swe_hip_adj *a, const State &in, const State &out, int64_t p, d
         ADJ_CHECK(a, swe::launch_finalize_dt(a->partial, 1024, cfl, c->G.dx, c->G.dy, c->scal,
                                              a->dts, a->counter, a->nsteps, c->stream));
     }
-    const int rc = recompute_u1_u2(a, in, p);
+    const int rc = stages_12(a, in, u1, u2, p);
     if (rc) return rc;
-    const State u2 = stage_state(a, p, 2);
     ADJ_CHECK(a, swe::launch_stage(3, c->bc, false, false, c->G,
                                    fwd_args(a, u2, in, out, a->dts + p), c->stream, nullptr));
     return SWE_OK;
+}
+
+int forward_step(swe_hip_adj *a, const State &in, const State &out, int64_t p, double cfl) {
+    return forward_step_via(a, in, scratch_stage(a, 1), scratch_stage(a, 2), out, p, cfl);
 }
 
 int advance(swe_hip_adj *a, const Action &act, double cfl) {
@@ -251,15 +303,11 @@ swe::AdjArgs adj_args(swe_hip_adj *a, const State &v, int lam_in, int lam_out,
     return r;
 }
 
-// Reverse step p from its start state in `src`: lam[0] = dL/dU^{p+1} -> dL/dU^p.
-int reverse_step(swe_hip_adj *a, int src, int64_t p) {
+// Reverse step p given its stage inputs U^p (base), U1, U2: lam[0] = dL/dU^{p+1} ->
+// dL/dU^p.
+int reverse_with(swe_hip_adj *a, const State &base, const State &u1, const State &u2,
+                 int64_t p) {
     swe_hip_ctx *c = a->ctx;
-    const State base = state_of(a, src);
-    if (!a->records) {  // with stage records the forward pass left U1, U2 of step p in place
-        const int rc = recompute_u1_u2(a, base, p);
-        if (rc) return rc;
-    }
-    const State u1 = stage_state(a, p, 1), u2 = stage_state(a, p, 2);
     unsigned char *m0 = a->mask[a->parity], *m1 = a->mask[1 - a->parity];
     // stage 3: V = U2, output U^{p+1} (mask m0)
     ADJ_CHECK(a, swe::launch_adjoint_stage(3, c->bc, c->G, adj_args(a, u2, 0, 1, m0, m1, p),
@@ -271,6 +319,79 @@ int reverse_step(swe_hip_adj *a, int src, int64_t p) {
     ADJ_CHECK(a, swe::launch_adjoint_stage(1, c->bc, c->G, adj_args(a, base, 2, 0, m0, m1, p),
                                            c->stream));
     a->parity = 1 - a->parity;
+    return SWE_OK;
+}
+
+// Binomial schedule: reverse step p from its start state in `src` (U1, U2 recomputed).
+int reverse_step(swe_hip_adj *a, int src, int64_t p) {
+    const State base = state_of(a, src);
+    const State u1 = scratch_stage(a, 1), u2 = scratch_stage(a, 2);
+    const int rc = stages_12(a, base, u1, u2, p);
+    if (rc) return rc;
+    return reverse_with(a, base, u1, u2, p);
+}
+
+// Records plan, forward: steps before the recorded tail keep a checkpoint every m steps
+// (stage states in the scratch buffers); the tail's steps write their records.
+int forward_records(swe_hip_adj *a, double cfl) {
+    swe_hip_ctx *c = a->ctx;
+    const int64_t N = a->nsteps, T = N - a->rec_steps, m = a->ckpt_every;
+    const State work = state_of(a, -1), cu{{c->U[0], c->U[1], c->U[2]}};
+    State in = T > 0 ? state_of(a, 0) : rec_state(a, 0, 0);
+    const size_t bytes = a->n_total * sizeof(double);
+    for (int k = 0; k < 3; ++k)
+        ADJ_CHECK(a, hipMemcpyAsync(in.p[k], c->U[k], bytes, hipMemcpyDeviceToDevice,
+                                    c->stream));
+    for (int64_t p = 0; p < N; ++p) {
+        const int64_t q = p + 1;
+        State u1, u2, out;
+        if (p < T) {
+            u1 = scratch_stage(a, 1);
+            u2 = scratch_stage(a, 2);
+            out = q == T ? rec_state(a, 0, 0) : (q % m == 0 ? state_of(a, (int)(q / m)) : work);
+        } else {
+            u1 = rec_state(a, p - T, 1);
+            u2 = rec_state(a, p - T, 2);
+            out = q < N ? rec_state(a, q - T, 0) : cu;
+        }
+        const int rc = forward_step_via(a, in, u1, u2, out, p, cfl);
+        if (rc) return rc;
+        in = out;
+    }
+    return SWE_OK;
+}
+
+// Records plan, backward: the recorded tail as stored, then each checkpoint segment
+// (last first) recomputed into the record buffer and reversed.
+int backward_records(swe_hip_adj *a) {
+    const int64_t N = a->nsteps, T = N - a->rec_steps, m = a->ckpt_every;
+    for (int64_t p = N - 1; p >= T; --p) {
+        const int rc = reverse_with(a, rec_state(a, p - T, 0), rec_state(a, p - T, 1),
+                                    rec_state(a, p - T, 2), p);
+        if (rc) return rc;
+    }
+    for (int64_t seg = a->nck - 1; seg >= 0; --seg) {
+        const int64_t s0 = seg * m, s1 = std::min(s0 + m, T);
+        const State ck = state_of(a, (int)seg);
+        auto base_of = [&](int64_t p) { return p == s0 ? ck : rec_state(a, p - s0, 0); };
+        for (int64_t p = s0; p < s1; ++p) {
+            const State u1 = rec_state(a, p - s0, 1), u2 = rec_state(a, p - s0, 2);
+            int rc = stages_12(a, base_of(p), u1, u2, p);
+            if (rc) return rc;
+            if (p + 1 < s1) {  // the segment's last step needs only its stage states
+                ADJ_CHECK(a, swe::launch_stage(3, a->ctx->bc, false, false, a->ctx->G,
+                                               fwd_args(a, u2, base_of(p),
+                                                        rec_state(a, p + 1 - s0, 0),
+                                                        a->dts + p),
+                                               a->ctx->stream, nullptr));
+            }
+        }
+        for (int64_t p = s1 - 1; p >= s0; --p) {
+            const int rc = reverse_with(a, base_of(p), rec_state(a, p - s0, 1),
+                                        rec_state(a, p - s0, 2), p);
+            if (rc) return rc;
+        }
+    }
     return SWE_OK;
 }
 
@@ -303,10 +424,16 @@ int backward_common(swe_hip_adj *a, const double *lh, const double *lhu, const d
     a->parity = 0;
     ADJ_CHECK(a, swe::launch_floor_mask(c->G, c->U[0], c->U[1], c->U[2], a->mask[0], c->stream));
     a->state = 2;
-    for (size_t i = a->first_adj; i < a->acts.size(); ++i) {
-        const Action &act = a->acts[i];
-        const int rc = act.kind == kAdvance ? advance(a, act, 0.0) : reverse_step(a, act.src, act.a);
+    if (a->records) {
+        const int rc = backward_records(a);
         if (rc) return rc;
+    } else {
+        for (size_t i = a->first_adj; i < a->acts.size(); ++i) {
+            const Action &act = a->acts[i];
+            const int rc =
+                act.kind == kAdvance ? advance(a, act, 0.0) : reverse_step(a, act.src, act.a);
+            if (rc) return rc;
+        }
     }
     double *gout[3] = {gh, ghu, ghv};
     for (int k = 0; k < 3; ++k)
@@ -339,8 +466,74 @@ int swe_hip_adj_schedule(int64_t nsteps, int32_t nsnap, int64_t *advances, int32
     return SWE_OK;
 }
 
-int swe_hip_adj_create(swe_hip_ctx *ctx, int64_t nsteps, int32_t nsnap, swe_hip_adj **out) {
-    if (!ctx || !out || nsteps < 1 || nsnap < -1) return SWE_ERR_INVALID;
+}  // extern "C"
+
+namespace {
+
+// Allocation shared by the create entry points: work buffers, adjoints, masks, dts.
+int adj_alloc_common(swe_hip_adj *a) {
+    const size_t bytes = a->n_total * sizeof(double);
+    for (int i = 0; i < 3; ++i)
+        if (hipMalloc(&a->W[i], bytes) != hipSuccess) return SWE_ERR_OOM;
+    for (auto &l : a->lam)
+        for (auto &p : l)
+            if (hipMalloc(&p, bytes) != hipSuccess) return SWE_ERR_OOM;
+    if (hipMalloc(&a->lam_b, bytes) != hipSuccess) return SWE_ERR_OOM;
+    for (auto &m : a->mask)
+        if (hipMalloc(&m, a->n_total) != hipSuccess) return SWE_ERR_OOM;
+    if (hipMalloc(&a->dts, sizeof(double) * (size_t)a->nsteps) != hipSuccess ||
+        hipMalloc(&a->partial, sizeof(double) * 1024) != hipSuccess ||
+        hipMalloc(&a->counter, sizeof(long long)) != hipSuccess)
+        return SWE_ERR_OOM;
+    return SWE_OK;
+}
+
+// Binomial checkpoints: nsnap states of 3 arrays and the schedule.
+int adj_setup_binomial(swe_hip_adj *a, int32_t nsnap) {
+    a->records = false;
+    a->nsnap = nsnap;
+    a->per_slot = 3;
+    if (hipMalloc(&a->slots, 3 * a->n_total * sizeof(double) * (size_t)nsnap) != hipSuccess)
+        return SWE_ERR_OOM;
+    a->acts = make_schedule(a->nsteps, nsnap);
+    for (size_t i = 0; i < a->acts.size(); ++i)
+        if (a->acts[i].kind == kAdjoint) {
+            a->first_adj = i;
+            break;
+        }
+    return SWE_OK;
+}
+
+// Records plan (K >= 1).
+int adj_setup_records(swe_hip_adj *a, const RecordPlan &pl) {
+    a->records = true;
+    a->rec_steps = pl.K;
+    a->ckpt_every = pl.m;
+    a->nck = pl.nck;
+    a->nsnap = (int32_t)std::min<int64_t>(INT32_MAX, pl.K + pl.nck);
+    a->per_slot = 3;
+    const size_t arr = a->n_total * sizeof(double);
+    if (hipMalloc(&a->rec, 9 * arr * (size_t)std::max(pl.K, pl.m)) != hipSuccess)
+        return SWE_ERR_OOM;
+    if (pl.nck > 0 && hipMalloc(&a->slots, 3 * arr * (size_t)pl.nck) != hipSuccess)
+        return SWE_ERR_OOM;
+    return SWE_OK;
+}
+
+// The storage plan for `budget` bytes of stored states: the records plan with the fewest
+// recomputed steps that fits, else binomial checkpoints in as many states as fit.
+int adj_setup_budget(swe_hip_adj *a, double budget) {
+    const double arr = (double)a->n_total * sizeof(double);
+    const int64_t arrays = (int64_t)std::min(budget / arr, 9.0 * (double)a->nsteps + 9.0);
+    const RecordPlan pl = plan_records(a->nsteps, arrays);
+    if (pl.K >= 1) return adj_setup_records(a, pl);
+    const int64_t fit = arrays / 3;
+    if (fit < 1) return SWE_ERR_OOM;
+    return adj_setup_binomial(a, (int32_t)std::min<int64_t>({a->nsteps, fit, INT32_MAX}));
+}
+
+int adj_begin(swe_hip_ctx *ctx, int64_t nsteps, swe_hip_adj **out, swe_hip_adj **made) {
+    if (!ctx || !out || nsteps < 1) return SWE_ERR_INVALID;
     *out = nullptr;
     if (ctx->G.ng != 2 || ctx->scheme != SWE_SCHEME_OPTIMIZED) return SWE_ERR_INVALID;
     if (hipSetDevice(ctx->device) != hipSuccess) return SWE_ERR_HIP;
@@ -348,54 +541,100 @@ int swe_hip_adj_create(swe_hip_ctx *ctx, int64_t nsteps, int32_t nsnap, swe_hip_
     a->ctx = ctx;
     a->nsteps = nsteps;
     a->n_total = ctx->n_total;
-    const size_t bytes = a->n_total * sizeof(double);
-    auto bad = [&](int code) {
-        swe_hip_adj_destroy(a);
-        return code;
-    };
-    for (int i = 0; i < 3; ++i)
-        if (hipMalloc(&a->W[i], bytes) != hipSuccess) return bad(SWE_ERR_OOM);
-    for (auto &l : a->lam)
-        for (auto &p : l)
-            if (hipMalloc(&p, bytes) != hipSuccess) return bad(SWE_ERR_OOM);
-    if (hipMalloc(&a->lam_b, bytes) != hipSuccess) return bad(SWE_ERR_OOM);
-    for (auto &m : a->mask)
-        if (hipMalloc(&m, a->n_total) != hipSuccess) return bad(SWE_ERR_OOM);
-    if (hipMalloc(&a->dts, sizeof(double) * (size_t)nsteps) != hipSuccess ||
-        hipMalloc(&a->partial, sizeof(double) * 1024) != hipSuccess ||
-        hipMalloc(&a->counter, sizeof(long long)) != hipSuccess)
-        return bad(SWE_ERR_OOM);
-    const size_t state_bytes = 3 * bytes;
-    if (nsnap == 0) {
-        // automatic: every step with its stage states if that fits in 60% of the free
-        // memory, else as many stored states as half of it holds
-        size_t free_b = 0, total_b = 0;
-        if (hipMemGetInfo(&free_b, &total_b) != hipSuccess) return bad(SWE_ERR_HIP);
-        if ((double)nsteps * 3.0 * (double)state_bytes <= 0.6 * (double)free_b &&
-            nsteps <= INT32_MAX) {
-            nsnap = -1;
+    *made = a;
+    return adj_alloc_common(a);
+}
+
+}  // namespace
+
+extern "C" {
+
+int swe_hip_adj_create(swe_hip_ctx *ctx, int64_t nsteps, int32_t nsnap, swe_hip_adj **out) {
+    if (nsnap < -1) return SWE_ERR_INVALID;
+    swe_hip_adj *a = nullptr;
+    int rc = adj_begin(ctx, nsteps, out, &a);
+    if (rc == SWE_OK) {
+        if (nsnap == 0) {
+            // automatic: the records plan (or binomial checkpoints) within 60% of the free
+            // device memory
+            size_t free_b = 0, total_b = 0;
+            rc = hipMemGetInfo(&free_b, &total_b) == hipSuccess
+                     ? adj_setup_budget(a, 0.6 * (double)free_b)
+                     : SWE_ERR_HIP;
+        } else if (nsnap == -1) {
+            RecordPlan all;
+            all.K = nsteps;
+            rc = adj_setup_records(a, all);
         } else {
-            const int64_t fit = (int64_t)(free_b / 2 / state_bytes);
-            nsnap = (int32_t)std::max<int64_t>(1, std::min<int64_t>({nsteps, fit, INT32_MAX}));
+            rc = adj_setup_binomial(a, (int32_t)std::min<int64_t>(nsnap, nsteps));
         }
     }
-    if (nsnap == -1) {
-        if (nsteps > INT32_MAX) return bad(SWE_ERR_INVALID);
-        nsnap = (int32_t)nsteps;
-        a->records = true;
-        a->per_slot = 9;
+    if (rc != SWE_OK) {
+        swe_hip_adj_destroy(a);
+        return rc;
     }
-    a->nsnap = nsnap;
-    if (hipMalloc(&a->slots, state_bytes * (size_t)(a->per_slot / 3) * (size_t)nsnap) !=
-        hipSuccess)
-        return bad(SWE_ERR_OOM);
-    a->acts = make_schedule(nsteps, nsnap);
-    for (size_t i = 0; i < a->acts.size(); ++i)
-        if (a->acts[i].kind == kAdjoint) {
-            a->first_adj = i;
-            break;
-        }
     *out = a;
+    return SWE_OK;
+}
+
+int swe_hip_adj_create_budget(swe_hip_ctx *ctx, int64_t nsteps, double max_bytes,
+                              swe_hip_adj **out) {
+    if (!(max_bytes > 0.0)) return SWE_ERR_INVALID;
+    swe_hip_adj *a = nullptr;
+    int rc = adj_begin(ctx, nsteps, out, &a);
+    if (rc == SWE_OK) rc = adj_setup_budget(a, max_bytes);
+    if (rc != SWE_OK) {
+        swe_hip_adj_destroy(a);
+        return rc;
+    }
+    *out = a;
+    return SWE_OK;
+}
+
+int swe_hip_adj_plan_budget(int64_t nsteps, int64_t n_total, double max_bytes,
+                            int64_t *record_steps, int64_t *ckpt_every, int64_t *stored_states) {
+    if (nsteps < 1 || n_total < 1 || !(max_bytes > 0.0)) return SWE_ERR_INVALID;
+    const double arr = (double)n_total * sizeof(double);
+    const int64_t arrays = (int64_t)std::min(max_bytes / arr, 9.0 * (double)nsteps + 9.0);
+    const RecordPlan pl = plan_records(nsteps, arrays);
+    int64_t K = pl.K, m = pl.m, states = pl.K + pl.nck;
+    if (K < 1) {  // binomial fallback
+        m = 0;
+        states = std::min<int64_t>({nsteps, arrays / 3, INT32_MAX});
+        if (states < 1) return SWE_ERR_OOM;
+    }
+    if (record_steps) *record_steps = K;
+    if (ckpt_every) *ckpt_every = m;
+    if (stored_states) *stored_states = states;
+    return SWE_OK;
+}
+
+int swe_hip_adj_plan(const swe_hip_adj *a, int64_t *record_steps, int64_t *ckpt_every,
+                     int64_t *stored_states, double *store_bytes, double *recomputed_steps) {
+    if (!a) return SWE_ERR_INVALID;
+    const double arr = (double)a->n_total * sizeof(double);
+    int64_t K = 0, m = 0, states = a->nsnap;
+    double bytes = 0.0, recomputed = 0.0;
+    if (a->records) {
+        K = a->rec_steps;
+        m = a->ckpt_every;
+        states = K + a->nck;
+        bytes = arr * (9.0 * (double)std::max(K, m) + 3.0 * (double)a->nck);
+        // each step before the tail once, the last step of every segment without stage 3
+        recomputed = (double)(a->nsteps - K) - (double)a->nck / 3.0;
+    } else {
+        int64_t adv = 0;
+        for (const Action &x : a->acts)
+            if (x.kind == kAdvance) adv += x.b - x.a;
+        bytes = arr * 3.0 * (double)a->nsnap;
+        // advances beyond the forward sweep, plus U1 and U2 of every reversed step
+        recomputed = (double)(adv - (a->nsteps - 1)) + 2.0 / 3.0 * (double)a->nsteps;
+    }
+    if (record_steps) *record_steps = K;
+    if (ckpt_every) *ckpt_every = m;
+    if (stored_states) *stored_states = states;
+    if (store_bytes) *store_bytes = bytes;
+    if (recomputed_steps) *recomputed_steps = recomputed;
     return SWE_OK;
 }
 
@@ -404,6 +643,7 @@ void swe_hip_adj_destroy(swe_hip_adj *a) {
     (void)hipSetDevice(a->ctx->device);
     (void)hipStreamSynchronize(a->ctx->stream);
     (void)hipFree(a->slots);
+    (void)hipFree(a->rec);
     for (auto *p : a->W) (void)hipFree(p);
     for (auto &l : a->lam)
         for (auto *p : l) (void)hipFree(p);
@@ -442,19 +682,23 @@ int swe_hip_adj_forward(swe_hip_adj *a, int32_t dt_mode, double value, double *d
         ADJ_CHECK(a, swe::launch_fill_int(reinterpret_cast<int *>(a->counter), 0, 2, c->stream));
     }
     ADJ_CHECK(a, swe::launch_fill_int(c->flag, 0, 1, c->stream));
-    // U^0 -> slot 0
-    for (int k = 0; k < 3; ++k)
-        ADJ_CHECK(a, hipMemcpyAsync(slot_ptr(a, 0, k), c->U[k], bytes, hipMemcpyDeviceToDevice,
-                                    c->stream));
-    // the schedule's first sweep: every action before the first reversal is an advance
-    for (size_t i = 0; i < a->first_adj; ++i) {
-        const int rc = advance(a, a->acts[i], cfl);
-        if (rc) return rc;
+    int rc = SWE_OK;
+    if (a->records) {
+        rc = forward_records(a, cfl);
+    } else {
+        // U^0 -> slot 0
+        for (int k = 0; k < 3; ++k)
+            ADJ_CHECK(a, hipMemcpyAsync(slot_ptr(a, 0, k), c->U[k], bytes,
+                                        hipMemcpyDeviceToDevice, c->stream));
+        // the schedule's first sweep: every action before the first reversal is an advance
+        for (size_t i = 0; i < a->first_adj && rc == SWE_OK; ++i) rc = advance(a, a->acts[i], cfl);
+        // the last step, from the start state the first reversal uses, into the context
+        if (rc == SWE_OK) {
+            const Action &first = a->acts[a->first_adj];
+            const State cu{{c->U[0], c->U[1], c->U[2]}};
+            rc = forward_step(a, state_of(a, first.src), cu, a->nsteps - 1, cfl);
+        }
     }
-    // the last step, from the start state the first reversal uses, into the context
-    const Action &first = a->acts[a->first_adj];
-    const State cu{{c->U[0], c->U[1], c->U[2]}};
-    int rc = forward_step(a, state_of(a, first.src), cu, a->nsteps - 1, cfl);
     if (rc) return rc;
     if (dt_hist && dt_mode != SWE_DT_GIVEN)
         ADJ_CHECK(a, hipMemcpyAsync(dt_hist, a->dts, sizeof(double) * (size_t)a->nsteps,

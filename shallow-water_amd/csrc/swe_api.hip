@@ -29,6 +29,9 @@ int fail(swe_hip_ctx *c, int code, const std::string &msg) {
     return code;
 }
 
+// Message of the calling thread's last failed swe_hip_create (no context survives it).
+thread_local std::string g_create_err;
+
 #define SWE_CHECK(ctx, call)                                                                 \
     do {                                                                                     \
         const hipError_t e__ = (call);                                                       \
@@ -197,7 +200,9 @@ const char *swe_hip_error_string(int code) {
     }
 }
 
-const char *swe_hip_last_error(const swe_hip_ctx *ctx) { return ctx ? ctx->err.c_str() : ""; }
+const char *swe_hip_last_error(const swe_hip_ctx *ctx) {
+    return ctx ? ctx->err.c_str() : g_create_err.c_str();
+}
 
 int swe_hip_device_count(void) {
     int n = 0;
@@ -206,16 +211,26 @@ int swe_hip_device_count(void) {
 }
 
 int swe_hip_create(const swe_grid_desc *d, int device, swe_hip_ctx **out) {
-    if (!d || !out) return SWE_ERR_INVALID;
+    g_create_err.clear();
+    auto reject = [](int code, const char *m) {
+        g_create_err = m;
+        return code;
+    };
+    if (!d || !out) return reject(SWE_ERR_INVALID, "swe_hip_create: null descriptor or output");
     *out = nullptr;
     if (d->ng < 2 || d->nx < d->ng || d->ny < d->ng || !(d->dx > 0.0) || !(d->dy > 0.0) ||
         !(d->g > 0.0) || (d->bc_kind != SWE_BC_REFLECTING && d->bc_kind != SWE_BC_TRANSMISSIVE))
-        return SWE_ERR_INVALID;
+        return reject(SWE_ERR_INVALID,
+                      "swe_hip_create: need ng >= 2, nx and ny >= ng, dx, dy, g > 0 and a "
+                      "known boundary kind");
     const long long nxt = (long long)d->nx + 2LL * d->ng, nyt = (long long)d->ny + 2LL * d->ng;
-    if (nxt * nyt >= (1LL << 31)) return SWE_ERR_INVALID;  // 32-bit cell indices
+    if (nxt * nyt >= (1LL << 31))  // 32-bit cell indices
+        return reject(SWE_ERR_INVALID, "swe_hip_create: (nx+2ng)*(ny+2ng) must be below 2^31");
     int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return SWE_ERR_NO_DEVICE;
-    if (device < 0 || device >= ndev) return SWE_ERR_INVALID;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return reject(SWE_ERR_NO_DEVICE, "swe_hip_create: no HIP device");
+    if (device < 0 || device >= ndev)
+        return reject(SWE_ERR_INVALID, "swe_hip_create: device index out of range");
 
     auto *c = new swe_hip_ctx();
     c->device = device;
@@ -236,7 +251,7 @@ int swe_hip_create(const swe_grid_desc *d, int device, swe_hip_ctx **out) {
 
     auto bad = [&](int code, const std::string &m) {
         swe_hip_destroy(c);
-        (void)m;
+        g_create_err = "swe_hip_create: " + m + " failed";
         return code;
     };
     if (hipSetDevice(device) != hipSuccess) return bad(SWE_ERR_HIP, "hipSetDevice");
@@ -396,7 +411,8 @@ int swe_hip_rhs(swe_hip_ctx *c, double *rh, double *rhu, double *rhv) {
 
 int swe_hip_set_tile_rows(swe_hip_ctx *c, int tile_rows) {
     if (!c || tile_rows < 1) return SWE_ERR_INVALID;
-    c->tile_rows = tile_rows;
+    // one segment covers every row; larger values would only overflow r0 + tile_rows
+    c->tile_rows = std::min(tile_rows, c->G.nx);
     return SWE_OK;
 }
 
@@ -428,6 +444,8 @@ int swe_hip_set_scheme(swe_hip_ctx *c, int scheme, int recon, int riemann) {
         return fail(c, SWE_ERR_INVALID, "unknown scheme");
     if (c->bc != SWE_BC_REFLECTING)
         return fail(c, SWE_ERR_INVALID, "the serial scheme family has reflecting walls only");
+    if (c->G.ng != 2)
+        return fail(c, SWE_ERR_INVALID, "the serial scheme family needs ghost width ng = 2");
     if (!swe::modular_supported(scheme == SWE_SCHEME_SERIAL, recon, riemann))
         return fail(c, SWE_ERR_INVALID, "Unsupported serial solver combination");
     c->scheme = scheme;
@@ -457,8 +475,8 @@ int swe_hip_time_steps(swe_hip_ctx *c, double dt, int64_t nsteps, double *stage_
     SWE_CHECK(c, hipSetDevice(c->device));
     SWE_CHECK(c, swe::launch_set_scalar(c->scal, dt, c->stream));
     SWE_CHECK(c, swe::launch_fill_int(c->flag, 0, 1, c->stream));
-    std::vector<hipEvent_t> ev((size_t)nsteps * 3 + 1);
-    for (auto &e : ev) SWE_CHECK(c, hipEventCreate(&e));
+    swe::EventSet ev((size_t)nsteps * 3 + 1);
+    SWE_CHECK(c, ev.status);
     int rc = SWE_OK;
     SWE_CHECK(c, hipEventRecord(ev[0], c->stream));
     for (int64_t s = 0; s < nsteps && rc == SWE_OK; ++s) {
@@ -488,7 +506,6 @@ int swe_hip_time_steps(swe_hip_ctx *c, double dt, int64_t nsteps, double *stage_
             for (int i = 0; i < 3; ++i) stage_ms[i] = acc[i] / (double)nsteps;
         if (total_ms) *total_ms = tot;
     }
-    for (auto &e : ev) (void)hipEventDestroy(e);
     if (rc) return rc;
     return check_flag(c);
 }

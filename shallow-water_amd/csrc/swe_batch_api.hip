@@ -76,9 +76,10 @@ void drop_graph(swe_hip_batch *b) {
 }
 
 int blocks_for(const swe_hip_batch *b) {
-    const int all = (b->ntiles + swe::kBlockWaves - 1) / swe::kBlockWaves;
+    const int bw = swe::kBatchBlockWaves;
+    const int all = (b->ntiles + bw - 1) / bw;
     if (b->waves_per_cu <= 0) return all;
-    const int want = (b->waves_per_cu * b->num_cus + swe::kBlockWaves - 1) / swe::kBlockWaves;
+    const int want = (b->waves_per_cu * b->num_cus + bw - 1) / bw;
     return std::max(1, std::min(all, want));
 }
 
@@ -471,9 +472,9 @@ int swe_hip_batch_autotune(swe_hip_batch *b, int *tile_rows_out, int *waves_per_
     // caller uploaded is untouched.  dt comes from the current per-instance dt array.
     const int tiles_c[] = {8, 16, 32, 64, 96, 128};
     const int waves_c[] = {8, 12, 16, 24, 0};
-    hipEvent_t e0, e1;
-    SWE_BCHECK(b, hipEventCreate(&e0));
-    SWE_BCHECK(b, hipEventCreate(&e1));
+    swe::EventSet ev(2);
+    SWE_BCHECK(b, ev.status);
+    hipEvent_t e0 = ev[0], e1 = ev[1];
     float best = 1e30f;
     int best_t = b->tile_rows, best_w = b->waves_per_cu;
     for (int t : tiles_c) {
@@ -502,8 +503,6 @@ int swe_hip_batch_autotune(swe_hip_batch *b, int *tile_rows_out, int *waves_per_
             }
         }
     }
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
     int rc = swe_hip_batch_set_launch(b, best_t, best_w);
     if (rc) return rc;
     if (tile_rows_out) *tile_rows_out = best_t;

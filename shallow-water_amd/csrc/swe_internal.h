@@ -3,8 +3,33 @@
 
 #include <hip/hip_runtime.h>
 #include <cstdint>
+#include <vector>
 
 namespace swe {
+
+// hipEvents created together and destroyed with the object, so that every return path of
+// a timing routine releases them.  `status` is the first creation error (hipSuccess if all
+// were created).
+struct EventSet {
+    std::vector<hipEvent_t> ev;
+    hipError_t status = hipSuccess;
+    explicit EventSet(size_t n) : ev(n, nullptr) {
+        for (auto &e : ev) {
+            const hipError_t r = hipEventCreate(&e);
+            if (r != hipSuccess) {
+                e = nullptr;
+                if (status == hipSuccess) status = r;
+            }
+        }
+    }
+    ~EventSet() {
+        for (auto e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+    EventSet(const EventSet &) = delete;
+    EventSet &operator=(const EventSet &) = delete;
+    hipEvent_t operator[](size_t i) const { return ev[i]; }
+};
 
 // One grid in the reference layout (omp_kernels.hpp:22-35): row-major, k = i*stride + j,
 // i = x index (slow), j = y index (contiguous), stride = ny + 2*ng.
@@ -33,6 +58,17 @@ struct StageArgs {
 constexpr int kWave = 64;
 constexpr int kStripCols = kWave - 4;  // output columns per wave (2-cell halo each side)
 constexpr int kBlockWaves = 4;
+// Waves per workgroup of the stage and batch kernels.  Their waves share nothing (no LDS,
+// no barrier in the march), so the workgroup size only decides how the dispatcher refills
+// the CUs.  Overridable at build time for tuning sweeps.
+#ifndef SWE_STAGE_BLOCK_WAVES
+#define SWE_STAGE_BLOCK_WAVES 4
+#endif
+#ifndef SWE_BATCH_BLOCK_WAVES
+#define SWE_BATCH_BLOCK_WAVES 4
+#endif
+constexpr int kStageBlockWaves = SWE_STAGE_BLOCK_WAVES;
+constexpr int kBatchBlockWaves = SWE_BATCH_BLOCK_WAVES;
 
 __host__ __device__ inline int strips_for(int ny) { return (ny + kStripCols - 1) / kStripCols; }
 
@@ -42,7 +78,8 @@ __host__ __device__ inline int strips_for(int ny) { return (ny + kStripCols - 1)
 // fast: reciprocal-based velocities (few-ulp differences from the reference arithmetic).
 hipError_t launch_stage(int stage, int bc, bool speed, bool fast, const GridDev &G,
                         const StageArgs &A, hipStream_t stream, dim3 *grid_out);
-dim3 stage_grid(const GridDev &G, int tile_rows);
+// Grid of a strip x row-segment launch with `block_waves` waves (strips) per workgroup.
+dim3 stage_grid(const GridDev &G, int tile_rows, int block_waves = kStageBlockWaves);
 
 // Modular serial scheme family (swe_modular.hip): bathy = the hydrostatic flux assembly
 // (HLL only, as the reference dispatches it), else the plain one; recon 0 MUSCL / 1
@@ -94,8 +131,8 @@ struct BatchArgs {
     int split_ok;
 };
 
-// Stage launch over the work-list: `blocks` blocks of 4 waves; wave w takes tiles
-// w, w + W, w + 2W, ... (W = 4*blocks), i.e. persistent when blocks*4 < ntiles.
+// Stage launch over the work-list: `blocks` blocks of kBatchBlockWaves waves; wave w takes
+// tiles w, w + W, w + 2W, ... (W = kBatchBlockWaves*blocks), i.e. persistent when W < ntiles.
 hipError_t launch_batch_stage(int stage, int bc, bool speed, bool fast, const BatchArgs &B,
                               int blocks, hipStream_t stream);
 // Max wave speed of each instance's current state (U = base.in_*) into max_bits.
@@ -131,7 +168,8 @@ struct AdjArgs {
     const double *dt;  // device scalar: this step's dt (frozen)
     int tile_rows;     // rows per wave (fused kernel) / x faces per wave (x-face kernel)
     int y_rows;        // rows per wave of the y-face kernel (two-kernel path)
-    int block_waves;   // waves per block of the fused kernel (4, 8, 16); 0: two kernels
+    int block_waves;   // waves per block of the fused kernel (2, 4, 8); 0: two kernels
+    int block_rows;    // fused kernel: most rows of a block (set by its launcher)
     int nstrips;       // 58-column strips across y (adj_strips_for)
     double *xf;        // two-kernel path: 8 x (nx+1) x stride adjoint outputs of the x faces
     unsigned char *xsel;  // two-kernel path: (nx+2) x stride minmod selections
@@ -141,6 +179,7 @@ int adj_strips_for(int ny);
 // LDS of one fused-kernel block; launches above kAdjMaxLds are rejected.
 constexpr size_t kAdjMaxLds = 160 * 1024;
 size_t adj_lds_bytes(int block_waves, int tile_rows);
+size_t adj_lds_rows_bytes(int block_rows);
 bool adj_launch_ok(int block_waves, int tile_rows);
 // ng must be 2.
 hipError_t launch_adjoint_stage(int stage, int bc, const GridDev &G, const AdjArgs &A,

@@ -505,7 +505,7 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
 
 template <int STAGE, int RECON, int RIEMANN, bool BATHY>
 hipError_t launch_mod(const GridDev &G, const StageArgs &A, hipStream_t stream, dim3 *grid_out) {
-    const dim3 grid = stage_grid(G, A.tile_rows);
+    const dim3 grid = stage_grid(G, A.tile_rows, kBlockWaves);
     if (grid_out) *grid_out = grid;
     hipLaunchKernelGGL((modular_stage_kernel<STAGE, RECON, RIEMANN, BATHY>), grid,
                        dim3(kWave * kBlockWaves), 0, stream, G, A);

@@ -298,19 +298,52 @@ __device__ __forceinline__ void run_tile(const GridDev &G, const StageArgs &A, l
     bad = m.bad;
 }
 
+// Workgroup -> tile order.  The dispatcher hands consecutive workgroup ids of a 1-D grid
+// to the XCDs in turn (id mod 8), and each XCD has its own L2.  xcd_order gives each XCD
+// runs of `chunk` consecutive tiles: workgroups b = x + 8k (XCD x) take tiles
+// G*8*chunk + x*chunk + k%chunk with G = k / chunk, so the tiles that run side by side on
+// one XCD are neighbours in the tile order (horizontal neighbours whose halo columns share
+// cache lines in that XCD's L2), while successive runs still rotate over the XCDs.  The
+// workgroups past the last whole round of 8*chunk keep their own index.  A bijection of
+// [0, total).
+constexpr int kXcds = 8;
+__device__ __forceinline__ int xcd_order(int b, int total, int chunk) {
+    const int span = kXcds * chunk;
+    if (chunk <= 1 || b >= (total / span) * span) return b;
+    const int x = b % kXcds, k = b / kXcds;
+    return (k / chunk) * span + x * chunk + k % chunk;
+}
+// The stage kernel's tiles all cost the same, so each XCD takes one eighth of the grid.
+#ifndef SWE_STAGE_XCD_REMAP
+#define SWE_STAGE_XCD_REMAP 1
+#endif
+__device__ __forceinline__ int xcd_tile(int b, int total) {
+    return SWE_STAGE_XCD_REMAP ? xcd_order(b, total, total / kXcds) : b;
+}
+// Batch tiles differ in length (full segments first, short last segments at the end of
+// the work-list): short runs keep the XCDs' shares of long and short tiles even.
+#ifndef SWE_BATCH_XCD_CHUNK
+#define SWE_BATCH_XCD_CHUNK 16
+#endif
+
 template <int STAGE, int BC, bool SPEED, bool FAST>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SWE_STAGE_WAVES_PER_EU)))
+__global__ __launch_bounds__(kWave * kStageBlockWaves)
+__attribute__((amdgpu_waves_per_eu(SWE_STAGE_WAVES_PER_EU)))
 void stage_kernel(const GridDev G, const StageArgs A) {
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
-    const int strip = blockIdx.x * kBlockWaves + wave;
+    // 1-D grid of (row segment, strip group) tiles, segment-major
+    const int groups = (A.nstrips + kStageBlockWaves - 1) / kStageBlockWaves;
+    const int t = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int seg = t / groups;
+    const int strip = (t - seg * groups) * kStageBlockWaves + wave;
 
     double speed = 0.0;
     int bad = 0;
 
     if (strip < A.nstrips)
         run_tile<STAGE, BC, SPEED, FAST>(G, A, 0, STAGE != 0 ? *A.dt : 0.0, strip,
-                                         G.ng + (int)blockIdx.y * A.tile_rows, lane, speed, bad);
+                                         G.ng + seg * A.tile_rows, lane, speed, bad);
 
     if (STAGE != 0) {
         if (__any(bad)) {
@@ -318,14 +351,18 @@ void stage_kernel(const GridDev G, const StageArgs A) {
         }
     }
     if (SPEED) {
-        __shared__ double red[kBlockWaves];
         for (int off = kWave / 2; off > 0; off >>= 1) speed = smax(speed, __shfl_xor(speed, off));
-        if (lane == 0) red[wave] = speed;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            double mx = red[0];
-            for (int w = 1; w < kBlockWaves; ++w) mx = smax(mx, red[w]);
-            A.partial[blockIdx.y * gridDim.x + blockIdx.x] = mx;
+        if constexpr (kStageBlockWaves == 1) {
+            if (lane == 0) A.partial[blockIdx.x] = speed;
+        } else {
+            __shared__ double red[kStageBlockWaves];
+            if (lane == 0) red[wave] = speed;
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                double mx = red[0];
+                for (int w = 1; w < kStageBlockWaves; ++w) mx = smax(mx, red[w]);
+                A.partial[blockIdx.x] = mx;
+            }
         }
     }
 }
@@ -347,10 +384,15 @@ __device__ __forceinline__ double wave_max(double v) {
 // speed goes to max_bits[inst] with a 64-bit atomic max on the bit pattern (exact for
 // non-negative doubles, order-independent); non-finite flags per instance.
 template <int STAGE, int BC, bool SPEED, bool FAST>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(SWE_BATCH_WAVES_PER_EU)))
+__global__ __launch_bounds__(kWave * kBatchBlockWaves)
+__attribute__((amdgpu_waves_per_eu(SWE_BATCH_WAVES_PER_EU)))
 void batch_stage_kernel(const BatchArgs B) {
     const int lane = threadIdx.x & (kWave - 1);
-    const int gw = __builtin_amdgcn_readfirstlane((int)((blockIdx.x * blockDim.x + threadIdx.x) / kWave));
+    // workgroups in XCD-aware order (xcd_tile), so neighbouring tiles of the work-list run
+    // on the same XCD
+    const int gw = __builtin_amdgcn_readfirstlane(
+        xcd_order((int)blockIdx.x, (int)gridDim.x, SWE_BATCH_XCD_CHUNK) * kBatchBlockWaves +
+        (int)(threadIdx.x / kWave));
     const int nw = (int)(gridDim.x * blockDim.x / kWave);
     for (int t = gw; t < B.ntiles; t += nw) {
         const BatchTile T = B.tiles[t];
@@ -415,7 +457,7 @@ __global__ __launch_bounds__(1024) void batch_finalize_kernel(const BatchInst *i
 template <int STAGE, int BC, bool SPEED, bool FAST>
 hipError_t launch_batch_one(const BatchArgs &B, int blocks, hipStream_t stream) {
     hipLaunchKernelGGL((batch_stage_kernel<STAGE, BC, SPEED, FAST>), dim3(blocks),
-                       dim3(kWave * kBlockWaves), 0, stream, B);
+                       dim3(kWave * kBatchBlockWaves), 0, stream, B);
     return hipGetLastError();
 }
 
@@ -535,7 +577,7 @@ __global__ void fill_double_kernel(double *dst, double v, long n) {
 
 template <int STAGE, int BC, bool SPEED, bool FAST>
 hipError_t launch_one(const GridDev &G, const StageArgs &A, hipStream_t stream, dim3 grid) {
-    hipLaunchKernelGGL((stage_kernel<STAGE, BC, SPEED, FAST>), grid, dim3(kWave * kBlockWaves), 0,
+    hipLaunchKernelGGL((stage_kernel<STAGE, BC, SPEED, FAST>), grid, dim3(kWave * kStageBlockWaves), 0,
                        stream, G, A);
     return hipGetLastError();
 }
@@ -556,15 +598,16 @@ hipError_t launch_bc(int stage, bool speed, const GridDev &G, const StageArgs &A
 
 }  // namespace
 
-dim3 stage_grid(const GridDev &G, int tile_rows) {
+dim3 stage_grid(const GridDev &G, int tile_rows, int block_waves) {
     const int nstrips = strips_for(G.ny);
-    return dim3((nstrips + kBlockWaves - 1) / kBlockWaves, (G.nx + tile_rows - 1) / tile_rows, 1);
+    return dim3((nstrips + block_waves - 1) / block_waves, (G.nx + tile_rows - 1) / tile_rows, 1);
 }
 
 hipError_t launch_stage(int stage, int bc, bool speed, bool fast, const GridDev &G,
                         const StageArgs &A, hipStream_t stream, dim3 *grid_out) {
     if (A.tile_rows < 1 || A.nstrips != strips_for(G.ny)) return hipErrorInvalidValue;
-    const dim3 grid = stage_grid(G, A.tile_rows);
+    const dim3 g2 = stage_grid(G, A.tile_rows);
+    const dim3 grid(g2.x * g2.y, 1, 1);  // segment-major tiles, see xcd_tile
     if (grid_out) *grid_out = grid;
     if (fast)
         return bc == 0 ? launch_bc<0, true>(stage, speed, G, A, stream, grid)
@@ -584,8 +627,8 @@ hipError_t launch_batch_stage(int stage, int bc, bool speed, bool fast, const Ba
 }
 
 hipError_t launch_batch_speed(const BatchArgs &B, int blocks, hipStream_t stream) {
-    hipLaunchKernelGGL(batch_speed_kernel, dim3(blocks), dim3(kWave * kBlockWaves), 0, stream,
-                       B);
+    hipLaunchKernelGGL(batch_speed_kernel, dim3(blocks), dim3(kWave * kBatchBlockWaves), 0,
+                       stream, B);
     return hipGetLastError();
 }
 

@@ -106,6 +106,14 @@ _SIGS = {
     "swe_hip_batch_set_graph": (C.c_int, [C.c_void_p, C.c_int]),
     "swe_hip_batch_sync": (C.c_int, [C.c_void_p]),
     "swe_hip_adj_create": (C.c_int, [C.c_void_p, C.c_int64, C.c_int32, C.POINTER(C.c_void_p)]),
+    "swe_hip_adj_create_budget": (C.c_int, [C.c_void_p, C.c_int64, C.c_double,
+                                            C.POINTER(C.c_void_p)]),
+    "swe_hip_adj_plan": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                   C.POINTER(C.c_int64), C.POINTER(C.c_double),
+                                   C.POINTER(C.c_double)]),
+    "swe_hip_adj_plan_budget": (C.c_int, [C.c_int64, C.c_int64, C.c_double,
+                                          C.POINTER(C.c_int64), C.POINTER(C.c_int64),
+                                          C.POINTER(C.c_int64)]),
     "swe_hip_adj_destroy": (None, [C.c_void_p]),
     "swe_hip_adj_last_error": (C.c_char_p, [C.c_void_p]),
     "swe_hip_adj_forward": (C.c_int, [C.c_void_p, C.c_int32, C.c_double, _dp]),
@@ -396,21 +404,40 @@ def adjoint_schedule(nsteps: int, nsnap: int):
     return adv.value, reps.value, acts.value
 
 
+def adjoint_plan_budget(nsteps: int, n_total: int, max_bytes: float):
+    """(recorded steps K, checkpoint interval m, stored states) of the storage plan
+    HipAdjoint(..., budget_bytes=max_bytes) picks; K = 0 means binomial checkpoints.
+    Host-only, needs no GPU."""
+    k, m, st = C.c_int64(), C.c_int64(), C.c_int64()
+    rc = lib().swe_hip_adj_plan_budget(int(nsteps), int(n_total), float(max_bytes), C.byref(k),
+                                       C.byref(m), C.byref(st))
+    if rc != SWE_OK:
+        raise SweError(rc, lib().swe_hip_error_string(rc).decode())
+    return k.value, m.value, st.value
+
+
 class HipAdjoint:
     """Discrete adjoint of `nsteps` SSP-RK3 steps of a HipSolver (swe_hip_adj_* in the C ABI).
 
     forward() runs the steps from the solver's current state, storing binomial checkpoints
-    (nsnap stored states, U^0 included; -1 = every step with its stage states; 0 = -1 when
-    that fits in 60% of the free device memory, else as many states as half of it holds)
+    (nsnap stored states, U^0 included; -1 = every step with its stage states; 0 = the
+    budget plan below for 60% of the free device memory) or, with budget_bytes, the plan
+    with the fewest recomputed steps in that much memory (the last steps' stage records
+    plus periodic checkpoints)
     and leaves U^N in the solver; backward(lam) takes dL/dU^N and returns
     dL/dU^0 and dL/dB with dt frozen from the forward pass.  One backward per forward."""
 
-    def __init__(self, solver: HipSolver, nsteps: int, nsnap: int = 0):
+    def __init__(self, solver: HipSolver, nsteps: int, nsnap: int = 0,
+                 budget_bytes: float | None = None):
         self.solver = solver
         self.nsteps = int(nsteps)
         self._a = C.c_void_p()
-        self._check(lib().swe_hip_adj_create(solver._ctx, self.nsteps, int(nsnap),
-                                             C.byref(self._a)))
+        if budget_bytes is not None:
+            self._check(lib().swe_hip_adj_create_budget(solver._ctx, self.nsteps,
+                                                        float(budget_bytes), C.byref(self._a)))
+        else:
+            self._check(lib().swe_hip_adj_create(solver._ctx, self.nsteps, int(nsnap),
+                                                 C.byref(self._a)))
 
     def _check(self, rc):
         if rc != SWE_OK:
@@ -471,6 +498,15 @@ class HipAdjoint:
         rows per wave of the y kernel).  0 keeps a value."""
         self._check(lib().swe_hip_adj_set_launch(self._a, int(block_waves), int(rows),
                                                  int(y_rows)))
+
+    def plan(self):
+        """Storage plan: recorded steps, checkpoint interval, stored states, their bytes and
+        the forward work the backward recomputes (in steps)."""
+        k, m, st, by, rc = C.c_int64(), C.c_int64(), C.c_int64(), C.c_double(), C.c_double()
+        self._check(lib().swe_hip_adj_plan(self._a, C.byref(k), C.byref(m), C.byref(st),
+                                           C.byref(by), C.byref(rc)))
+        return {"record_steps": k.value, "ckpt_every": m.value, "stored_states": st.value,
+                "store_bytes": by.value, "recomputed_steps": rc.value}
 
     def info(self):
         n, r, f, b = C.c_int32(), C.c_int32(), C.c_double(), C.c_double()

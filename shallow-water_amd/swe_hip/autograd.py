@@ -52,11 +52,40 @@ class SweRollout(torch.autograd.Function):
         return out[0], out[1], out[2], out[3], None, None, None, None, None, None
 
 
+def _plain_rollout(h0, hu0, hv0, B, grid: dict, nsteps: int, dt, cfl, dts):
+    """The same steps without an adjoint: no checkpoint store is allocated."""
+    for t in (h0, hu0, hv0, B):
+        if t.dtype != torch.float64 or not t.is_cuda:
+            raise TypeError("swe_rollout needs float64 device tensors")
+    dev = h0.device.index or 0
+    h0, hu0, hv0, B = (t.detach().contiguous() for t in (h0, hu0, hv0, B))
+    with HipSolver(grid["nx"], grid["ny"], grid["dx"], grid["dy"], grid["g"],
+                   grid.get("bc", 0), 2, dev) as solver:
+        torch.cuda.synchronize(h0.device)
+        solver.upload_device(h0, hu0, hv0, B)
+        if dts is not None:
+            for d in dts:
+                solver.step(float(d), 1)
+        elif cfl is not None:
+            solver.step_adaptive(float(cfl), nsteps)
+        elif nsteps > 0:
+            solver.step(float(dt), nsteps)
+        outs = [torch.empty_like(h0) for _ in range(3)]
+        solver.download_device(*outs)
+    return tuple(outs)
+
+
 def swe_rollout(h0, hu0, hv0, B, grid: dict, nsteps: int, dt=None, cfl=None, dts=None,
                 nsnap: int = 0):
-    """Differentiable N-step rollout; exactly one of dt / cfl / dts."""
+    """Differentiable N-step rollout; exactly one of dt / cfl / dts.  When no gradient can
+    be asked for (grad mode off, or no input requires grad) the steps run without the
+    adjoint's checkpoint store."""
     if sum(x is not None for x in (dt, cfl, dts)) != 1:
         raise ValueError("give exactly one of dt, cfl, dts")
+    if dts is not None and len(dts) != int(nsteps):
+        raise ValueError("dts must hold nsteps values")
+    if not (torch.is_grad_enabled() and any(t.requires_grad for t in (h0, hu0, hv0, B))):
+        return _plain_rollout(h0, hu0, hv0, B, grid, int(nsteps), dt, cfl, dts)
     return SweRollout.apply(h0, hu0, hv0, B, grid, int(nsteps), dt, cfl, dts, int(nsnap))
 
 

@@ -46,13 +46,14 @@ def setup(case, bc):
     return gr, B, dt
 
 
-def gpu_gradient(hip, case, bc, nsteps, dt, lam, nsnap=2, launch=None):
-    """launch: (block_waves, rows, y_rows) for HipAdjoint.set_launch, None = default."""
+def gpu_gradient(hip, case, bc, nsteps, dt, lam, nsnap=2, launch=None, budget=None):
+    """launch: (block_waves, rows, y_rows) for HipAdjoint.set_launch, None = default;
+    budget: bytes for the budget storage plan (instead of nsnap)."""
     c = dict(case)
     c["bc"] = bc
     with hip.HipSolver.from_case(c) as s:
         s.upload(case["h"], case["hu"], case["hv"], case["B"])
-        with hip.HipAdjoint(s, nsteps, nsnap) as adj:
+        with hip.HipAdjoint(s, nsteps, nsnap, budget_bytes=budget) as adj:
             if launch:
                 adj.set_launch(*launch)
             adj.forward(dt=dt)
@@ -158,6 +159,20 @@ def test_gradient_independent_of_checkpoints(hip):
         else:
             for a, b in zip(g, ref):
                 assert same(a, b), nsnap
+    # budget plans: records of the last K steps plus checkpoints every m steps, down to the
+    # binomial fallback (budgets in arrays of one field)
+    arr = case["h"].size * 8
+    plans = set()
+    for arrays in (81, 80, 60, 45, 33, 27, 24, 20, 15, 9, 6, 3):
+        with hip.HipSolver.from_case(case) as s, \
+                hip.HipAdjoint(s, 9, budget_bytes=arrays * arr) as adj:
+            p = adj.plan()
+        plans.add((p["record_steps"], p["ckpt_every"]))
+        assert p["store_bytes"] <= arrays * arr
+        _, g = gpu_gradient(hip, case, 0, 9, dt, lam, budget=arrays * arr)
+        for a, b in zip(g, ref):
+            assert same(a, b), (arrays, p)
+    assert len(plans) >= 6 and (9, 1) in plans and any(k == 0 for k, _ in plans)
 
 
 @pytest.mark.parametrize("bc", [0, 1])
@@ -241,6 +256,36 @@ def test_torch_autograd_wrapper(hip):
     L.backward()
     for k, want in zip(("h", "hu", "hv", "B"), (gh, ghu, ghv, gb)):
         assert same(t[k].grad.cpu().numpy(), want), k
+
+
+def test_torch_rollout_without_grad_skips_the_adjoint(hip):
+    """Under no_grad, or with no input requiring grad, swe_rollout runs the plain solver
+    (no checkpoint store) and returns the same bits as the differentiable path."""
+    torch = pytest.importorskip("torch")
+    from swe_hip.autograd import swe_rollout
+
+    case = mild_state(24, 70, 21)
+    _, _, dt = setup(case, 0)
+    dev = torch.device("cuda:0")
+    grid = {"nx": case["nx"], "ny": case["ny"], "dx": case["dx"], "dy": case["dy"],
+            "g": case["g"], "bc": 0}
+    t = {k: torch.tensor(case[k], dtype=torch.float64, device=dev, requires_grad=True)
+         for k in ("h", "hu", "hv", "B")}
+    want = swe_rollout(t["h"], t["hu"], t["hv"], t["B"], grid, 5, dt=dt, nsnap=3)
+    assert all(o.requires_grad for o in want)
+    with torch.no_grad():
+        got = swe_rollout(t["h"], t["hu"], t["hv"], t["B"], grid, 5, dt=dt, nsnap=3)
+    plain = {k: v.detach() for k, v in t.items()}
+    got2 = swe_rollout(plain["h"], plain["hu"], plain["hv"], plain["B"], grid, 5, dt=dt)
+    for outs in (got, got2):
+        assert not any(o.requires_grad for o in outs)
+        for a, b in zip(outs, want):
+            assert same(a.cpu().numpy(), b.detach().cpu().numpy())
+    dts = [dt * 0.5] * 5
+    a = swe_rollout(plain["h"], plain["hu"], plain["hv"], plain["B"], grid, 5, dts=dts)
+    b = swe_rollout(t["h"], t["hu"], t["hv"], t["B"], grid, 5, dts=dts, nsnap=3)
+    for x, y in zip(a, b):
+        assert same(x.cpu().numpy(), y.detach().cpu().numpy())
 
 
 @pytest.mark.slow

@@ -320,6 +320,39 @@ def test_lake_4096_full_size_properties(hip, bump):
 
 
 @pytest.mark.slow
+@pytest.mark.parametrize("bump", [True, False])
+def test_lake_4096_wetdry_as_stated_stepwise(hip, bump):
+    """Config 2 exactly as stated: 4096^2 on [0,10]^2, 32 seeded Gaussian bumps, lake at rest
+    eta = 0.5 (h = max(0, eta - B), 23% of the cells dry) with and without the 0.05 bump,
+    transmissive walls, CFL 0.4 dt.  The HIP path and the oracle advance one step at a time:
+    the full arrays are bitwise equal after every step, until the step at which the
+    reference scheme produces a non-finite state (omp_kernels.hpp:309-331: at wet/dry
+    fronts MUSCL on eta gives dry cells a face depth and the hydrostatic correction gives
+    them momentum, so u = hu/h blows up once h is just above the 1e-8 floor).  At that same
+    step the HIP path reports SWE_ERR_NONFINITE."""
+    c = W.lake_wet_dry(4096, seed=2, bump=bump)
+    gr, h, hu, hv, B = oracle_state(c)
+    dt, _ = oracle.stable_dt(gr, h, hu, hv, 0.4)
+    failed_at = None
+    with gpu_solver(hip, c) as s:
+        for step in range(1, 201):
+            rc, _, _ = oracle.run(gr, h, hu, hv, B, 1, dt=dt)
+            if rc != 0:
+                with pytest.raises(hip.SweError) as e:
+                    s.step(dt, 1)
+                assert e.value.code == hip.SWE_ERR_NONFINITE
+                failed_at = step
+                break
+            s.step(dt, 1)
+            got = s.download()
+            for comp, g_, w_ in zip(("h", "hu", "hv"), got, (h, hu, hv)):
+                assert_bitwise(g_, w_, f"config 2 (bump={bump}) step {step} {comp}")
+    assert failed_at is not None, "the reference scheme no longer diverges within 200 steps"
+    print(f"config 2 (bump={bump}): bitwise for {failed_at - 1} steps, non-finite at step "
+          f"{failed_at} in both")
+
+
+@pytest.mark.slow
 def test_dam_break_8192_full_run_properties(hip):
     """Config 3 at its real size for all its 1000 steps (CFL 0.45 dt, reflecting walls):
     the state stays finite, the depth stays between the two initial levels up to the wave

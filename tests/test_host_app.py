@@ -2,9 +2,12 @@
 reference's schema plus the "HIP" backend key, validation, the initial-state generators and
 the NetCDF classic output, all without a GPU (--print-config, --initial-state).
 
-Generators are pinned by their closed forms (the reference's formulas, evaluated here with
-numpy); the output files are read back with scipy.io.netcdf_file.
+Generators are pinned bitwise to the reference's own setup code (apply_initial_condition /
+apply_bathymetry compiled from the reference tree into oracle/_ref; golden states in
+tests/golden/setup_states.npz) and by their closed forms; the output files are read back with
+scipy.io.netcdf_file.
 """
+import json
 import os
 import subprocess
 
@@ -181,3 +184,64 @@ def test_other_initial_conditions(tmp_path, ic):
 def test_no_bathymetry(tmp_path):
     d, attrs, _ = read_initial(tmp_path, bathymetry__type='"None"')
     assert not d["B"].any() and attrs["bathymetry"] == b"None"
+
+
+GOLDEN = os.path.join(REPO, "tests", "golden")
+SETUPS = json.load(open(os.path.join(GOLDEN, "setup_states.json")))
+
+
+def setup_overrides(case):
+    """TOML overrides that make the app's configuration the golden case's."""
+    m = case["mesh"]
+    over = {"mesh__Nx": m["nx"], "mesh__Ny": m["ny"], "mesh__nG": m["ng"],
+            "mesh__Lx": repr(float(m["lx"])), "mesh__Ly": repr(float(m["ly"]))}
+    for table, key in (("initial_condition", "initial_condition"), ("bathymetry", "bathymetry")):
+        for k, v in case[key].items():
+            over[f"{table}__{k}"] = f'"{v}"' if k == "type" else repr(float(v))
+    if case["bathymetry"]["type"] != "GaussHill":
+        # the app validates the hill's centre even when unused; keep it inside the domain
+        over["bathymetry__bathy_x0"] = repr(float(m["lx"]) / 2)
+        over["bathymetry__bathy_y0"] = repr(float(m["ly"]) / 2)
+    if case["initial_condition"]["type"] != "GaussInitial":
+        over["initial_condition__sigma_x"] = "1.0"
+        over["initial_condition__sigma_y"] = "1.0"
+    return over
+
+
+@pytest.mark.parametrize("name", sorted(SETUPS))
+def test_initial_state_bitwise_vs_reference_setup(tmp_path, name):
+    """The app's initial arrays (the NetCDF snapshot 0 and B, interior cells as the
+    reference writes them) equal, bit for bit, the reference's own generators for every
+    initial-condition x bathymetry type and for the TOML forms of configs 1 and 3."""
+    case = SETUPS[name]
+    d, _, _ = read_initial(tmp_path, **setup_overrides(case))
+    z = np.load(os.path.join(GOLDEN, "setup_states.npz"))
+    ng = case["mesh"]["ng"]
+    inner = (slice(ng, -ng), slice(ng, -ng))
+    for k, got in (("h", d["h"][0]), ("hu", d["hu"][0]), ("hv", d["hv"][0]), ("B", d["B"])):
+        want = z[f"{name}__{k}"][inner]
+        assert got.shape == want.shape, (name, k)
+        got = np.ascontiguousarray(got, dtype=np.float64)  # NetCDF data is big-endian
+        assert np.array_equal(got.view(np.uint64),
+                              np.ascontiguousarray(want).view(np.uint64)), (name, k)
+
+
+def test_setup_fixtures_are_the_reference_setup_code():
+    """The golden initial states are what the reference's compiled setup code produces now
+    (pins the fixtures; needs oracle/_ref)."""
+    import sys
+
+    sys.path.insert(0, REPO)
+    import oracle
+
+    if not oracle.ref_available():
+        pytest.skip("reference harness not built (oracle/_ref)")
+    z = np.load(os.path.join(GOLDEN, "setup_states.npz"))
+    for name, case in SETUPS.items():
+        m = case["mesh"]
+        ic = dict(case["initial_condition"])
+        bt = dict(case["bathymetry"])
+        arrays = oracle.ref_initial_state(m["nx"], m["ny"], m["ng"], m["lx"], m["ly"],
+                                          ic.pop("type"), ic, bt.pop("type"), bt)
+        for k, a in zip(("h", "hu", "hv", "B"), arrays):
+            assert np.array_equal(a.view(np.uint64), z[f"{name}__{k}"].view(np.uint64)), (name, k)

@@ -156,3 +156,31 @@ def test_schedule_is_optimal():
         assert adv == binomial_optimum(n, s)
     # store-all: every step advanced exactly once (the first sweep)
     assert adjoint_schedule(2000, 2000)[:2] == (1999, 1)
+
+
+def test_record_plan_is_the_best_that_fits():
+    """The budget plan (the last K steps' stage records, a checkpoint every m steps before
+    them) maximises K within the budget: brute force over every (K, m) on small cases, in
+    units of one array (n_total = 1 cell, 8 bytes)."""
+    from swe_hip import adjoint_plan_budget
+
+    def need(n, K, m):
+        return 9 * max(K, m) + 3 * (-(-(n - K) // m))
+
+    for n in range(1, 40):
+        for arrays in list(range(3, 12 * n, 7)) + [9 * n, 9 * n + 5]:
+            K, m, states = adjoint_plan_budget(n, 1, 8.0 * arrays)
+            if 9 * n <= arrays:
+                assert (K, states) == (n, n), (n, arrays)
+                continue
+            best = max((k for k in range(1, n) for mm in range(1, n + 1)
+                        if need(n, k, mm) <= arrays), default=0)
+            assert K == best, (n, arrays, K, best)
+            if K:
+                assert need(n, K, m) <= arrays and states == K + -(-(n - K) // m)
+            else:  # binomial fallback: as many 3-array states as fit
+                assert states == min(n, arrays // 3)
+    # config 5 at a quarter of the store-every-step memory: about 500 of 2000 steps recorded
+    n_total = 1028 * 1028
+    K, m, _ = adjoint_plan_budget(2000, n_total, 0.25 * 2000 * 9 * n_total * 8)
+    assert 480 <= K <= 500 and m >= 1
