@@ -133,6 +133,17 @@ int swe_hip_step(swe_hip_ctx *ctx, double dt, int64_t nsteps);
  * NULL) receives the nsteps dt values used. */
 int swe_hip_step_adaptive(swe_hip_ctx *ctx, double cfl, int64_t nsteps, double *dt_hist);
 
+/* swe_hip_step_adaptive that allocates nothing: no counterpart in the reference; for callers
+ * that cannot allocate device memory (code under HIP graph capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  Same kernels and results.  The dt history goes to
+ * dt_hist_device, a caller-owned device array of nsteps doubles (NULL: not recorded); the
+ * CFL reduction's per-block maxima and the history counter live in a fixed scratch array
+ * of the library (one per GPU, held until the process ends), so the stage launch may have
+ * at most 65536 blocks (SWE_ERR_INVALID with a message beyond that).  Steps are launched
+ * directly (no graph).  Calls that overlap on one GPU share that scratch and race. */
+int swe_hip_step_adaptive_noalloc(swe_hip_ctx *ctx, double cfl, int64_t nsteps,
+                                  double *dt_hist_device);
+
 /* compute_rhs of the current state (omp_solver.hpp:220-248): rhs arrays of n_total on the
  * host, interior written, ghosts 0 (as the reference's zero-initialized vectors). */
 int swe_hip_rhs(swe_hip_ctx *ctx, double *rhs_h, double *rhs_hu, double *rhs_hv);

@@ -11,6 +11,7 @@
 #include <algorithm>
 #include <cstdio>
 #include <cstring>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -156,6 +157,31 @@ int run_steps(swe_hip_ctx *c, int mode, double cfl, int64_t nsteps) {
         c->graph_cfl = cfl;
     }
     for (int64_t s = 0; s < nsteps; ++s) SWE_CHECK(c, hipGraphLaunch(c->graph, c->stream));
+    return SWE_OK;
+}
+
+// ---- scratch of the allocation-free adaptive step ------------------------------------
+// Per-block maxima of the CFL wave-speed reduction (kNoAllocPartials doubles) followed by
+// the dt-history counter, in a file-scope device array: each GPU the library runs on holds
+// one copy (516 KiB) until the process ends.  Host code never passes the array's name to a
+// kernel (in host code it is a stand-in, not a device address); it uses the address
+// hipGetSymbolAddress returns for the current device, looked up once per device.
+constexpr int kNoAllocPartials = 65536;
+__device__ __align__(256) double scratch[kNoAllocPartials + 32];
+constexpr int kMaxDevices = 64;
+void *g_scratch_ptr[kMaxDevices] = {};
+std::mutex g_scratch_mutex;
+
+int scratch_for_device(swe_hip_ctx *c, double **out) {
+    if (c->device < 0 || c->device >= kMaxDevices)
+        return fail(c, SWE_ERR_INVALID, "device index beyond the scratch table");
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[c->device]) {
+        void *ptr = nullptr;
+        SWE_CHECK(c, hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));  // device is current
+        g_scratch_ptr[c->device] = ptr;
+    }
+    *out = static_cast<double *>(g_scratch_ptr[c->device]);
     return SWE_OK;
 }
 
@@ -390,6 +416,49 @@ int swe_hip_step_adaptive(swe_hip_ctx *c, double cfl, int64_t nsteps, double *dt
     if (dt_hist && nsteps > 0)
         SWE_CHECK(c, hipMemcpyAsync(dt_hist, c->hist, sizeof(double) * (size_t)nsteps,
                                     hipMemcpyDeviceToHost, c->stream));
+    return check_flag(c);
+}
+
+int swe_hip_step_adaptive_noalloc(swe_hip_ctx *c, double cfl, int64_t nsteps,
+                                  double *dt_hist_device) {
+    if (!c || nsteps < 0 || !(cfl > 0.0)) return SWE_ERR_INVALID;
+    if (c->scheme != SWE_SCHEME_OPTIMIZED)
+        return fail(c, SWE_ERR_INVALID, "the serial scheme family runs with a fixed dt");
+    SWE_CHECK(c, hipSetDevice(c->device));
+    const dim3 g2 = swe::stage_grid(c->G, c->tile_rows);
+    const long long blocks = (long long)g2.x * g2.y;
+    if (blocks > kNoAllocPartials)
+        return fail(c, SWE_ERR_INVALID,
+                    "swe_hip_step_adaptive_noalloc: the stage launch has " +
+                        std::to_string(blocks) + " blocks, more than the " +
+                        std::to_string(kNoAllocPartials) +
+                        " per-block maxima its fixed scratch holds (use a taller tile or "
+                        "swe_hip_step_adaptive)");
+    double *partial = nullptr;
+    int rc = scratch_for_device(c, &partial);
+    if (rc) return rc;
+    long long *counter = reinterpret_cast<long long *>(partial + kNoAllocPartials);
+    const long long hist_cap = dt_hist_device ? nsteps : 0;
+    // the counter and the flag start at 0 (fill kernels, no memset node in a capture)
+    SWE_CHECK(c, swe::launch_fill_int(reinterpret_cast<int *>(counter), 0, 2, c->stream));
+    SWE_CHECK(c, swe::launch_fill_int(c->flag, 0, 1, c->stream));
+    // dt of the first step from U^0
+    SWE_CHECK(c, swe::launch_speed_reduce(c->G, c->U[0], c->U[1], c->U[2], partial, 1024,
+                                          c->stream));
+    SWE_CHECK(c, swe::launch_finalize_dt(partial, 1024, cfl, c->G.dx, c->G.dy, c->scal,
+                                         dt_hist_device, counter, hist_cap, c->stream));
+    for (int64_t s = 0; s < nsteps; ++s) {
+        for (int stage = 1; stage <= 3; ++stage) {
+            StageArgs a = stage_args(c, stage);
+            a.partial = partial;
+            dim3 grid;
+            SWE_CHECK(c, launch_ctx_stage(c, stage, stage == 3, a, &grid));
+            if (stage == 3)
+                SWE_CHECK(c, swe::launch_finalize_dt(partial, (int)(grid.x * grid.y), cfl,
+                                                     c->G.dx, c->G.dy, c->scal, dt_hist_device,
+                                                     counter, hist_cap, c->stream));
+        }
+    }
     return check_flag(c);
 }
 

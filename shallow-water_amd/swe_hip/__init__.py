@@ -78,6 +78,8 @@ _SIGS = {
     "swe_hip_stable_dt": (C.c_int, [C.c_void_p, C.c_double, _dp, _dp]),
     "swe_hip_step": (C.c_int, [C.c_void_p, C.c_double, C.c_int64]),
     "swe_hip_step_adaptive": (C.c_int, [C.c_void_p, C.c_double, C.c_int64, _dp]),
+    "swe_hip_step_adaptive_noalloc": (C.c_int, [C.c_void_p, C.c_double, C.c_int64,
+                                                C.c_void_p]),
     "swe_hip_rhs": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "swe_hip_set_tile_rows": (C.c_int, [C.c_void_p, C.c_int]),
     "swe_hip_get_tile_rows": (C.c_int, [C.c_void_p]),
@@ -247,6 +249,17 @@ class HipSolver:
         hist = np.zeros(max(int(nsteps), 1))
         self._check(lib().swe_hip_step_adaptive(self._ctx, float(cfl), int(nsteps), _p(hist)))
         return hist[:nsteps]
+
+    def step_adaptive_noalloc(self, cfl, nsteps=1, dt_hist=None):
+        """swe_hip_step_adaptive_noalloc: no device allocation; dt_hist is None or a float64
+        device tensor of at least nsteps entries (written on the device)."""
+        ptr = None
+        if dt_hist is not None:
+            if str(dt_hist.dtype) != "torch.float64" or not dt_hist.is_cuda or \
+                    dt_hist.numel() < int(nsteps) or not dt_hist.is_contiguous():
+                raise TypeError("dt_hist must be a contiguous float64 device tensor of nsteps")
+            ptr = C.c_void_p(dt_hist.data_ptr())
+        self._check(lib().swe_hip_step_adaptive_noalloc(self._ctx, float(cfl), int(nsteps), ptr))
 
     def rhs(self):
         out = [np.empty(self.shape) for _ in range(3)]
