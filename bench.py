@@ -39,6 +39,7 @@ sys.path.insert(0, os.path.join(REPO, "shallow-water_amd"))
 import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0            # MI355X HBM3E peak, GB/s
+SHADER_CLOCK_GHZ = 2.4           # MI355X peak engine clock
 BYTES_PER_CELL_STAGE = (56, 80, 80)  # algorithmic HBM bytes per interior cell, stages 1..3
 
 
@@ -50,7 +51,7 @@ def mass(h, ng):
     return math.fsum(h[ng:-ng, ng:-ng].ravel().tolist())
 
 
-def cpu_baseline(case, steps, dt):
+def cpu_baseline(case, steps, dt, workload="dam_break"):
     """The reference's compiled OpenMP kernels (oracle/_ref; the oracle port when the
     reference harness is absent) on `steps` fixed-dt steps of the same grid, in a child
     process with one OpenMP thread per available physical core, OMP_PROC_BIND=true and
@@ -59,7 +60,7 @@ def cpu_baseline(case, steps, dt):
     from oracle import cpu_baseline as cb
 
     host = cb.host_cpus()
-    r = cb.run_child(["--workload", "dam_break", "--n", str(case["nx"]), "--steps", str(steps),
+    r = cb.run_child(["--workload", workload, "--n", str(case["nx"]), "--steps", str(steps),
                       "--dt", float(dt).hex()], host["threads"], timeout=900)
     if r["rc"] != 0:
         raise RuntimeError("CPU baseline run failed")
@@ -67,7 +68,7 @@ def cpu_baseline(case, steps, dt):
     info = {
         "value": cells * steps / r["seconds"] / 1e9, "unit": "Gcell-updates/s",
         "cores": r["threads"], "kind": r["kind"],
-        "sample": f"{case['nx']}x{case['ny']} dam break, {steps} fixed-dt SSP-RK3 steps "
+        "sample": f"{case['nx']}x{case['ny']} {workload.replace('_', ' ')}, {steps} fixed-dt SSP-RK3 steps "
                   f"({'oracle/_ref: reference omp_kernels.hpp' if r['kind'] == 'reference' else 'oracle port'}"
                   f", OpenMP, g++ -O3 -ffp-contract=off), {r['seconds']:.2f} s",
         "threads": r["threads"], "binding": f"OMP_PROC_BIND={r['bind']} OMP_PLACES={r['places']}",
@@ -204,7 +205,7 @@ def ensemble_bench(steps, base_steps, check_every=32):
     }
 
 
-def adjoint_bench(n=1024, steps=2000):
+def adjoint_bench(n=1024, steps=2000, plans=None):
     """Config 5: 1024^2, 2000 SSP-RK3 steps (fixed dt from CFL 0.4 at t = 0), gauge-misfit
     loss at 16 seeded cells.  Wall time of the plain forward (graph-replayed steps) vs the
     adjoint (forward with checkpoints + backward), for the automatic storage choice and for a
@@ -232,6 +233,8 @@ def adjoint_bench(n=1024, steps=2000):
     runs = [("auto", 0, None), ("records_25pct", 0, 0.25 * store_all),
             ("records_10pct", 0, 0.10 * store_all)]
     runs += [(f"binomial_{k}", k, None) for k in (64, 128, 256, 512, steps)]
+    if plans:
+        runs = [r for r in runs if r[0] in plans]
     ref_grad = None
     for label, nsnap, budget in runs:
         with swe_hip.HipSolver.from_case(case) as s:
@@ -262,7 +265,7 @@ def adjoint_bench(n=1024, steps=2000):
             np.array_equal(a.view(np.uint64), b.view(np.uint64)) for a, b in zip(g, ref_grad))
         log(f"[bench] adjoint {label}: {t_adj:.3f} s vs forward {t_fwd:.3f} s "
             f"-> {t_adj / t_fwd:.2f}x ({info}, {plan})")
-    res["ratio"] = res["auto"]["ratio"]
+    res["ratio"] = res["auto"]["ratio"] if "auto" in res else None
     # the least stored memory among the runs that stay within 4x the forward time
     ok = [(v["store_frac_of_store_all"], k) for k, v in res.items()
           if isinstance(v, dict) and v.get("ratio", 99) <= 4.0]
@@ -355,6 +358,13 @@ def main():
     ap.add_argument("--no-serial", action="store_true")
     ap.add_argument("--serial-n", type=int, default=4096)
     ap.add_argument("--adjoint-steps", type=int, default=2000)
+    ap.add_argument("--adjoint-plans", type=str, default="",
+                    help="comma-separated subset of the adjoint storage plans to run")
+    ap.add_argument("--workload", choices=("dam_break", "wet_bathymetry"), default="dam_break",
+                    help="dam_break: the headline config (flat bottom); wet_bathymetry: the "
+                         "same size with 16 Gaussian bumps and background flow (sloped faces)")
+    ap.add_argument("--precision", choices=("exact", "fast"), default="exact",
+                    help="fast: SWE_PRECISION_FAST (not bitwise; never the headline)")
     args = ap.parse_args()
 
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -369,13 +379,15 @@ def main():
         raise SystemExit("no HIP device visible")
 
     n = args.n
-    log(f"[bench] generating dam break {n}^2")
-    case = W.dam_break(n)
+    log(f"[bench] generating {args.workload} {n}^2")
+    case = W.dam_break(n) if args.workload == "dam_break" else W.adjoint_case(n)
     s = swe_hip.HipSolver.from_case(case)
     if args.tile_rows:
         s.tile_rows = args.tile_rows
+    if args.precision == "fast":
+        s.precision = swe_hip.PRECISION_FAST
     s.upload(case["h"], case["hu"], case["hv"], case["B"])
-    dt, max_speed = s.stable_dt(0.45)
+    dt, max_speed = s.stable_dt(0.45 if args.workload == "dam_break" else 0.4)
     m0 = mass(case["h"], case["ng"])
     log(f"[bench] dt={dt!r} max_speed={max_speed!r} tile_rows={s.tile_rows}")
 
@@ -401,13 +413,23 @@ def main():
     ach_dom = stage_bytes[dom] / (stage_ms[dom] * 1e-3) / 1e9
     step_bytes = sum(stage_bytes)
     ach_step = step_bytes / (sum(stage_ms) * 1e-3) / 1e9
-    traffic = None
+    traffic, valu = None, None
     tf = os.path.join(REPO, "profiles", "stage_kernel_dam_break_8192_traffic.json")
-    if os.path.exists(tf) and n == 8192:
-        try:
-            traffic = json.load(open(tf)).get("stage2_hbm_bytes_per_launch")
-        except Exception:
-            traffic = None
+    if os.path.exists(tf) and n == 8192 and args.workload == "dam_break" and \
+            args.precision == "exact":
+        prof = json.load(open(tf))
+        traffic = prof.get("stage2_hbm_bytes_per_launch")
+        qc = prof.get("stage2_valu_active_quad_cycles_per_launch")
+        if qc:
+            # the VALU issue floor: the stage-2 kernel's VALU quad-cycles (SQ_ACTIVE_INST_VALU,
+            # transcendental passes included) spread over 256 CUs x 4 SIMDs at the rated clock
+            floor_ms = qc * 4 / (256 * 4) / (SHADER_CLOCK_GHZ * 1e9) * 1e3
+            valu = {"active_quad_cycles_per_launch": qc,
+                    "insts_per_launch": prof.get("stage2_valu_insts_per_launch"),
+                    "issue_floor_ms": round(floor_ms, 4), "clock_ghz": SHADER_CLOCK_GHZ,
+                    "issue_frac": round(floor_ms / stage_ms[dom], 4),
+                    "valu_busy_frac_counters": prof.get("stage2_valu_busy_frac"),
+                    "source": "profiles/stage_kernel_dam_break_8192_pmc.md"}
 
     out = {
         "metric": "FP64 Gcell-updates/s @8192^2 (dam break, SSP-RK3 MUSCL+HLL+HR)",
@@ -422,13 +444,19 @@ def main():
         "vs_baseline": None,
         "dtype": "f64",
         "data": "synthetic (seeded dam break)",
-        "config": {"workload": f"dam_break_{n}x{n}_fp64_reflecting_cfl0.45", "nx": n, "ny": n,
-                   "ng": 2, "dt": dt, "parallelism": "single GPU",
-                   "tile_rows": s.tile_rows},
+        "config": {"workload": (f"dam_break_{n}x{n}_fp64_reflecting_cfl0.45"
+                                if args.workload == "dam_break" else
+                                f"wet_bathymetry_{n}x{n}_fp64_reflecting_cfl0.4"),
+                   "nx": n, "ny": n, "ng": 2, "dt": dt, "parallelism": "single GPU",
+                   "tile_rows": s.tile_rows, "precision": args.precision},
         "roofline": {
             "bound": "hbm", "kernel": "stage_kernel<2> (SSP-RK3 stage 2, 80 B/cell)",
             "achieved": round(ach_dom, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach_dom / HBM_PEAK_GBS, 4), "traffic": traffic,
+            # the roof that binds: FP64 VALU issue (the HBM fraction above is the metric's
+            # pricing; the kernel is limited by its bitwise FP64 arithmetic, not by bytes)
+            "binding_roof": "fp64_valu_issue" if valu else None,
+            "valu": valu,
         },
         "roofline_step": {
             "bytes_per_cell_update": sum(BYTES_PER_CELL_STAGE), "achieved": round(ach_step, 1),
@@ -444,14 +472,14 @@ def main():
         from oracle import cpu_baseline as cb
 
         log(f"[bench] CPU baseline: {args.cpu_steps} steps")
-        info, ref_sha = cpu_baseline(case, args.cpu_steps, dt)
+        info, ref_sha = cpu_baseline(case, args.cpu_steps, dt, args.workload)
         # same steps on the GPU, compared bitwise (sha256 of the full arrays)
         with swe_hip.HipSolver.from_case(case) as s2:
             s2.upload(case["h"], case["hu"], case["hv"], case["B"])
             s2.step(dt, args.cpu_steps)
             got = s2.download()
         same = cb.digest(*got) == ref_sha
-        out["cpu_parity_bitwise"] = bool(same)
+        out["cpu_parity_bitwise"] = bool(same) if args.precision == "exact" else None
         log(f"[bench] cpu {info['value']:.4f} Gcell/s ({info['cores']} threads, "
             f"{info['binding']}, host {info['host']}), bitwise={same}")
         if not args.no_c1:
@@ -461,7 +489,8 @@ def main():
     if not args.no_ensemble:
         out["batched"] = ensemble_bench(args.ensemble_steps, args.ensemble_base_steps)
     if not args.no_adjoint:
-        out["adjoint"] = adjoint_bench(steps=args.adjoint_steps)
+        out["adjoint"] = adjoint_bench(steps=args.adjoint_steps,
+                                       plans=[p for p in args.adjoint_plans.split(",") if p])
     if not args.no_serial:
         out["serial_schemes"] = serial_bench(n=args.serial_n)
     print(json.dumps(out), flush=True)

@@ -296,12 +296,13 @@ int swe_hip_adj_backward_device(swe_hip_adj *adj, const double *lam_h, const dou
                                 double *grad_hv, double *grad_b);
 
 /* Launch shape of the adjoint stages.  block_waves 2, 4 or 8: one fused kernel per stage,
- * blocks of that many waves with `rows` rows each (default 4 x 3; the block's LDS,
- * (block_waves*rows + 3) * 4 KiB, must stay within 160 KiB).  block_waves 0: the two-kernel
- * path (x-face kernel with `rows` faces per wave writing to device memory, then the
- * y-face kernel with `y_rows` rows per wave), whose buffers (9 doubles per cell) are
- * allocated on first selection.  0 for rows / y_rows keeps the current value.  Every shape
- * gives the same bits. */
+ * blocks of that many waves with at most `rows` rows each (default 4 x 4; the block's LDS,
+ * (block_waves*rows + 3) * 4 KiB, must stay within 160 KiB).  The launch splits the rows
+ * into blocks of equal height, as many as fill the device's resident blocks in the fewest
+ * rounds.  block_waves 0: the two-kernel path (x-face kernel with `rows` faces per wave
+ * writing to device memory, then the y-face kernel with `y_rows` rows per wave), whose
+ * buffers (9 doubles per cell) are allocated on first selection.  0 for rows / y_rows keeps
+ * the current value.  Every shape gives the same bits. */
 int swe_hip_adj_set_launch(swe_hip_adj *adj, int block_waves, int rows, int y_rows);
 
 /* The checkpoint schedule for (nsteps, nsnap), computed on the host (no device needed):

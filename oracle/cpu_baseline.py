@@ -113,7 +113,8 @@ def digest(*arrays) -> list[str]:
 
 def _main():
     ap = argparse.ArgumentParser()
-    ap.add_argument("--workload", choices=("dam_break", "gauss_pulse"), required=True)
+    ap.add_argument("--workload", choices=("dam_break", "gauss_pulse", "wet_bathymetry"),
+                    required=True)
     ap.add_argument("--n", type=int, required=True)
     ap.add_argument("--steps", type=int, required=True)
     ap.add_argument("--dt", type=str, default=None, help="fixed dt as float.hex()")
@@ -131,6 +132,8 @@ def _main():
 
     if a.workload == "dam_break":
         c = W.dam_break(a.n) if a.g is None else W.dam_break(a.n, g=a.g)
+    elif a.workload == "wet_bathymetry":
+        c = W.adjoint_case(a.n) if a.g is None else W.adjoint_case(a.n, g=a.g)
     else:
         c = W.gauss_pulse(a.n) if a.g is None else W.gauss_pulse(a.n, g=a.g)
     gr = oracle.grid(c["nx"], c["ny"], c["ng"], c["dx"], c["dy"], g=c["g"], bc=c["bc"])

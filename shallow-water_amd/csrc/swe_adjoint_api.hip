@@ -22,9 +22,12 @@
 // recorded steps as they are, then, segment by segment from the last, recomputes the m
 // steps of a segment from its checkpoint into the record buffer (stage 3 of the segment's
 // last step is not needed) and reverses them.  Every step before the recorded tail is
-// recomputed exactly once.  K = nsteps is "store every step".  plan_records picks m and K
-// for a memory budget: the largest K (fewest recomputed steps) whose records, max(K, m)
-// x 9 arrays, and checkpoints, ceil((nsteps - K) / m) x 3 arrays, fit.
+// recomputed exactly once.  With two segments or more, the record buffer holds two segment
+// buffers and a second stream recomputes the next segment into one while the first stream
+// reverses the other (events order them), so the forward kernels fill the CUs the adjoint
+// kernels leave idle.  K = nsteps is "store every step".  plan_records picks m and K for a
+// memory budget: the largest K (fewest recomputed steps) whose records, max(K, m or 2m) x
+// 9 arrays, and checkpoints, ceil((nsteps - K) / m) x 3 arrays, fit.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -103,7 +106,10 @@ std::vector<Action> make_schedule(int64_t nsteps, int32_t nsnap) {
 // Records plan (see the header): K recorded steps, a checkpoint every m steps before them.
 struct RecordPlan {
     int64_t K = 0, m = 1, nck = 0;
-    int64_t arrays() const { return 9 * std::max(K, m) + 3 * nck; }
+    // record slots: the tail's K, and two segment buffers when there are two segments or
+    // more (one recomputed while the other is reversed)
+    int64_t slots() const { return std::max(K, nck >= 2 ? 2 * m : m); }
+    int64_t arrays() const { return 9 * slots() + 3 * nck; }
 };
 
 int64_t ceil_div(int64_t a, int64_t b) { return (a + b - 1) / b; }
@@ -117,9 +123,16 @@ RecordPlan plan_records(int64_t nsteps, int64_t arrays_budget) {
         return best;
     }
     for (int64_t m = 1; m <= nsteps; ++m) {
-        auto need = [&](int64_t K) { return 9 * std::max(K, m) + 3 * ceil_div(nsteps - K, m); };
-        if (need(std::min(m, nsteps - 1)) > arrays_budget) continue;
-        int64_t lo = std::min(m, nsteps - 1), hi = nsteps - 1;  // need() rises with K
+        auto need = [&](int64_t K) {
+            const int64_t nck = ceil_div(nsteps - K, m);
+            return 9 * std::max(K, nck >= 2 ? 2 * m : m) + 3 * nck;
+        };
+        // need() does not rise with K below kt = min(2m, nsteps - m) (the segment buffers
+        // dominate the records while fewer steps are recorded) and does not fall above it
+        int64_t kt = std::min(2 * m, nsteps - m);
+        kt = std::max<int64_t>(1, std::min(kt, nsteps - 1));
+        if (need(kt) > arrays_budget) continue;
+        int64_t lo = kt, hi = nsteps - 1;
         while (lo < hi) {
             const int64_t mid = lo + (hi - lo + 1) / 2;
             if (need(mid) <= arrays_budget)
@@ -146,7 +159,10 @@ struct swe_hip_adj {
     int64_t rec_steps = 0;    // records plan: K recorded steps (the last K of the run)
     int64_t ckpt_every = 1;   // records plan: checkpoint interval m before them
     int64_t nck = 0;          // records plan: ceil((nsteps - K) / m) checkpoints
-    double *rec = nullptr;    // records plan: max(K, m) records of 9 arrays (U^p, U1, U2)
+    int64_t rec_slots = 0;    // records plan: record slots (RecordPlan::slots())
+    double *rec = nullptr;    // records plan: RecordPlan::slots() records of 9 arrays
+                              // (U^p, U1, U2)
+    hipStream_t stream2 = nullptr;  // records plan: the segment recompute stream
     int per_slot = 3;
     double *W[3] = {nullptr, nullptr, nullptr};
     double *lam[4][3] = {};   // step adjoint, dL/dU2, dL/dU1, U^n accumulator
@@ -160,7 +176,7 @@ struct swe_hip_adj {
     std::vector<Action> acts;
     size_t first_adj = 0;
     int block_waves = 4;  // fused kernel: waves per block (0 selects the two-kernel path)
-    int fused_rows = 3;   // fused kernel: rows per wave
+    int fused_rows = 4;   // fused kernel: at most this many rows per wave
     int tile_rows = 4;    // two-kernel path: x faces per wave
     int y_rows = 4;       // two-kernel path: rows per wave of the y-face kernel
     int state = 0;  // 0 created, 1 forward done (checkpoints valid), 2 consumed
@@ -230,13 +246,15 @@ State rec_state(swe_hip_adj *a, int64_t r, int which) {
     return State{{base, base + a->n_total, base + 2 * a->n_total}};
 }
 
-// Stages 1 and 2 of step p from `base` into u1, u2.
-int stages_12(swe_hip_adj *a, const State &base, const State &u1, const State &u2, int64_t p) {
+// Stages 1 and 2 of step p from `base` into u1, u2 (on `st`; default the context's stream).
+int stages_12(swe_hip_adj *a, const State &base, const State &u1, const State &u2, int64_t p,
+              hipStream_t st = nullptr) {
     swe_hip_ctx *c = a->ctx;
+    if (!st) st = c->stream;
     ADJ_CHECK(a, swe::launch_stage(1, c->bc, false, false, c->G,
-                                   fwd_args(a, base, base, u1, a->dts + p), c->stream, nullptr));
+                                   fwd_args(a, base, base, u1, a->dts + p), st, nullptr));
     ADJ_CHECK(a, swe::launch_stage(2, c->bc, false, false, c->G,
-                                   fwd_args(a, u1, base, u2, a->dts + p), c->stream, nullptr));
+                                   fwd_args(a, u1, base, u2, a->dts + p), st, nullptr));
     return SWE_OK;
 }
 
@@ -362,36 +380,101 @@ int forward_records(swe_hip_adj *a, double cfl) {
 }
 
 // Records plan, backward: the recorded tail as stored, then each checkpoint segment
-// (last first) recomputed into the record buffer and reversed.
+// (last first) recomputed into a segment buffer of the record buffer and reversed.  With
+// two segments or more the recompute runs on stream2, one segment ahead of the reversal,
+// alternating between two segment buffers at the top of the record buffer (slots
+// [R - m, R) and [R - 2m, R - m), R = its slot count).  The tail is reversed from its top
+// slot down, so the first two recomputes start as soon as the tail's reversal has left
+// their buffer, while the rest of the tail is still being reversed.
 int backward_records(swe_hip_adj *a) {
-    const int64_t N = a->nsteps, T = N - a->rec_steps, m = a->ckpt_every;
+    swe_hip_ctx *c = a->ctx;
+    const int64_t N = a->nsteps, T = N - a->rec_steps, m = a->ckpt_every, K = a->rec_steps;
+    const int64_t R = a->rec_slots;
+    const bool overlap = a->nck >= 2 && a->stream2;
+    // events: 0/1 buffer b left by the tail, 2/3 buffer b recomputed, 4/5 buffer b
+    // reversed, 6 stream2 done
+    swe::EventSet ev(overlap ? 7 : 0);
+    if (overlap) ADJ_CHECK(a, ev.status);
+    auto off = [&](int b) { return overlap ? R - (int64_t)(b + 1) * m : 0; };
+    for (int b = 0; b < 2 && overlap; ++b)
+        if (off(b) >= K) ADJ_CHECK(a, hipEventRecord(ev[b], c->stream));  // never used by the tail
     for (int64_t p = N - 1; p >= T; --p) {
         const int rc = reverse_with(a, rec_state(a, p - T, 0), rec_state(a, p - T, 1),
                                     rec_state(a, p - T, 2), p);
         if (rc) return rc;
+        for (int b = 0; b < 2 && overlap; ++b)
+            if (p - T == std::max<int64_t>(off(b), 0) && off(b) < K)
+                ADJ_CHECK(a, hipEventRecord(ev[b], c->stream));
     }
-    for (int64_t seg = a->nck - 1; seg >= 0; --seg) {
-        const int64_t s0 = seg * m, s1 = std::min(s0 + m, T);
+    // segment seg: steps [s0, s1), records from slot o
+    auto seg_range = [&](int64_t seg, int64_t &s0, int64_t &s1) {
+        s0 = seg * m;
+        s1 = std::min(s0 + m, T);
+    };
+    auto recompute = [&](int64_t seg, int64_t o, hipStream_t st) -> int {
+        int64_t s0, s1;
+        seg_range(seg, s0, s1);
         const State ck = state_of(a, (int)seg);
-        auto base_of = [&](int64_t p) { return p == s0 ? ck : rec_state(a, p - s0, 0); };
         for (int64_t p = s0; p < s1; ++p) {
-            const State u1 = rec_state(a, p - s0, 1), u2 = rec_state(a, p - s0, 2);
-            int rc = stages_12(a, base_of(p), u1, u2, p);
+            const State base = p == s0 ? ck : rec_state(a, o + p - s0, 0);
+            const State u1 = rec_state(a, o + p - s0, 1), u2 = rec_state(a, o + p - s0, 2);
+            const int rc = stages_12(a, base, u1, u2, p, st);
             if (rc) return rc;
-            if (p + 1 < s1) {  // the segment's last step needs only its stage states
-                ADJ_CHECK(a, swe::launch_stage(3, a->ctx->bc, false, false, a->ctx->G,
-                                               fwd_args(a, u2, base_of(p),
-                                                        rec_state(a, p + 1 - s0, 0),
+            if (p + 1 < s1)  // the segment's last step needs only its stage states
+                ADJ_CHECK(a, swe::launch_stage(3, c->bc, false, false, c->G,
+                                               fwd_args(a, u2, base,
+                                                        rec_state(a, o + p + 1 - s0, 0),
                                                         a->dts + p),
-                                               a->ctx->stream, nullptr));
-            }
+                                               st, nullptr));
         }
+        return SWE_OK;
+    };
+    auto reverse = [&](int64_t seg, int64_t o) -> int {
+        int64_t s0, s1;
+        seg_range(seg, s0, s1);
+        const State ck = state_of(a, (int)seg);
         for (int64_t p = s1 - 1; p >= s0; --p) {
-            const int rc = reverse_with(a, base_of(p), rec_state(a, p - s0, 1),
-                                        rec_state(a, p - s0, 2), p);
+            const State base = p == s0 ? ck : rec_state(a, o + p - s0, 0);
+            const int rc = reverse_with(a, base, rec_state(a, o + p - s0, 1),
+                                        rec_state(a, o + p - s0, 2), p);
             if (rc) return rc;
         }
+        return SWE_OK;
+    };
+    if (!overlap) {
+        for (int64_t seg = a->nck - 1; seg >= 0; --seg) {
+            int rc = recompute(seg, 0, c->stream);
+            if (!rc) rc = reverse(seg, 0);
+            if (rc) return rc;
+        }
+        return SWE_OK;
     }
+    hipStream_t s2 = a->stream2;
+    const int64_t S = a->nck;
+    ADJ_CHECK(a, hipStreamWaitEvent(s2, ev[0], 0));
+    int rc = recompute(S - 1, off(0), s2);
+    if (rc) return rc;
+    ADJ_CHECK(a, hipEventRecord(ev[2], s2));
+    for (int64_t seg = S - 1; seg >= 0; --seg) {
+        const int b = (int)((S - 1 - seg) % 2);
+        if (seg >= 1) {
+            const int nb = 1 - b;
+            // buffer nb: first use waits for the tail to leave it, later ones for the
+            // reversal of the segment it last held (seg + 1)
+            ADJ_CHECK(a, hipStreamWaitEvent(s2, seg + 1 <= S - 1 ? ev[4 + nb] : ev[nb], 0));
+            rc = recompute(seg - 1, off(nb), s2);
+            if (rc) return rc;
+            ADJ_CHECK(a, hipEventRecord(ev[2 + nb], s2));
+        }
+        ADJ_CHECK(a, hipStreamWaitEvent(c->stream, ev[2 + b], 0));
+        rc = reverse(seg, off(b));
+        if (rc) return rc;
+        ADJ_CHECK(a, hipEventRecord(ev[4 + b], c->stream));
+    }
+    ADJ_CHECK(a, hipEventRecord(ev[6], s2));
+    ADJ_CHECK(a, hipStreamWaitEvent(c->stream, ev[6], 0));
+    // the events are released on return: drain both streams' use of them first
+    ADJ_CHECK(a, hipStreamSynchronize(c->stream));
     return SWE_OK;
 }
 
@@ -510,11 +593,13 @@ int adj_setup_records(swe_hip_adj *a, const RecordPlan &pl) {
     a->rec_steps = pl.K;
     a->ckpt_every = pl.m;
     a->nck = pl.nck;
+    a->rec_slots = pl.slots();
     a->nsnap = (int32_t)std::min<int64_t>(INT32_MAX, pl.K + pl.nck);
     a->per_slot = 3;
     const size_t arr = a->n_total * sizeof(double);
-    if (hipMalloc(&a->rec, 9 * arr * (size_t)std::max(pl.K, pl.m)) != hipSuccess)
-        return SWE_ERR_OOM;
+    if (hipMalloc(&a->rec, 9 * arr * (size_t)pl.slots()) != hipSuccess) return SWE_ERR_OOM;
+    if (pl.nck >= 2 && hipStreamCreateWithFlags(&a->stream2, hipStreamNonBlocking) != hipSuccess)
+        return SWE_ERR_HIP;
     if (pl.nck > 0 && hipMalloc(&a->slots, 3 * arr * (size_t)pl.nck) != hipSuccess)
         return SWE_ERR_OOM;
     return SWE_OK;
@@ -619,7 +704,11 @@ int swe_hip_adj_plan(const swe_hip_adj *a, int64_t *record_steps, int64_t *ckpt_
         K = a->rec_steps;
         m = a->ckpt_every;
         states = K + a->nck;
-        bytes = arr * (9.0 * (double)std::max(K, m) + 3.0 * (double)a->nck);
+        RecordPlan pl;
+        pl.K = K;
+        pl.m = m;
+        pl.nck = a->nck;
+        bytes = arr * (double)pl.arrays();
         // each step before the tail once, the last step of every segment without stage 3
         recomputed = (double)(a->nsteps - K) - (double)a->nck / 3.0;
     } else {
@@ -644,6 +733,10 @@ void swe_hip_adj_destroy(swe_hip_adj *a) {
     (void)hipStreamSynchronize(a->ctx->stream);
     (void)hipFree(a->slots);
     (void)hipFree(a->rec);
+    if (a->stream2) {
+        (void)hipStreamSynchronize(a->stream2);
+        (void)hipStreamDestroy(a->stream2);
+    }
     for (auto *p : a->W) (void)hipFree(p);
     for (auto &l : a->lam)
         for (auto *p : l) (void)hipFree(p);

@@ -163,7 +163,7 @@ def test_gradient_independent_of_checkpoints(hip):
     # binomial fallback (budgets in arrays of one field)
     arr = case["h"].size * 8
     plans = set()
-    for arrays in (81, 80, 60, 45, 33, 27, 24, 20, 15, 9, 6, 3):
+    for arrays in (81, 75, 66, 57, 48, 45, 39, 27, 20, 9, 3):
         with hip.HipSolver.from_case(case) as s, \
                 hip.HipAdjoint(s, 9, budget_bytes=arrays * arr) as adj:
             p = adj.plan()
@@ -172,7 +172,10 @@ def test_gradient_independent_of_checkpoints(hip):
         _, g = gpu_gradient(hip, case, 0, 9, dt, lam, budget=arrays * arr)
         for a, b in zip(g, ref):
             assert same(a, b), (arrays, p)
-    assert len(plans) >= 6 and (9, 1) in plans and any(k == 0 for k, _ in plans)
+    # store-all, records with one segment, records with 2+ segments (overlapped recompute
+    # on the second stream) and binomial checkpoints
+    assert len(plans) >= 7 and (9, 1) in plans and any(k == 0 for k, _ in plans)
+    assert any(k and -(-(9 - k) // m) >= 2 for k, m in plans)
 
 
 @pytest.mark.parametrize("bc", [0, 1])
@@ -288,13 +291,11 @@ def test_torch_rollout_without_grad_skips_the_adjoint(hip):
         assert same(x.cpu().numpy(), y.detach().cpu().numpy())
 
 
-@pytest.mark.slow
-def test_config5_finite_differences(hip):
-    """1024^2, 2000 steps, gauge-misfit loss: the adjoint's directional derivatives against
-    central differences of the GPU forward on 8 smooth random directions in (eta0, b).  The
-    discrete map is only piecewise smooth (minmod, wave-speed and branch kinks everywhere),
-    so agreement is limited by kink crossings; the bound here is what that allows."""
-    n, steps = 1024, 2000
+def config5_fd_errors(hip, steps, eps):
+    """Relative |FD - adjoint| of the config-5 gauge-misfit loss (1024^2, the first `steps`
+    steps) on 8 smooth random directions in (eta0, b): central differences of the GPU
+    forward with half-width eps against the adjoint's directional derivative."""
+    n = 1024
     case = W.adjoint_case(n, steps=steps)
     rows, cols = W.adjoint_gauges(n)
     with hip.HipSolver.from_case(case) as s:
@@ -338,11 +339,32 @@ def test_config5_finite_differences(hip):
             dirs.append(a)
         de, db = dirs[0], 0.3 * dirs[1]
         gv = float((g_eta * de).sum() + (g_b * db).sum())
-        eps = 1e-8
         lp = float(np.sum((run(case["h"] + eps * (de - db), case["B"] + eps * db) - target) ** 2))
         lm = float(np.sum((run(case["h"] - eps * (de - db), case["B"] - eps * db) - target) ** 2))
         fd = (lp - lm) / (2 * eps)
         errs.append(abs(fd - gv) / abs(gv))
+    return errs
+
+
+@pytest.mark.parametrize("steps,median_bar,max_bar", [(10, 1e-6, 1e-6), (50, 1e-6, 5e-6)])
+def test_config5_finite_differences_short_rollouts(hip, steps, median_bar, max_bar):
+    """Where central differences resolve the derivative to the 1e-6 bar: the config-5
+    workload over its first 10 steps (all 8 directions within 1e-6; measured max 8e-8) and
+    50 steps (median within 1e-6; measured 1.3e-7, max 1.5e-6), eps = 1e-8.  Over more
+    steps the kink crossings of the piecewise-smooth scheme dominate the FD error (median
+    2.7e-6 at 200 steps, 1.5e-5 at 2000)."""
+    errs = config5_fd_errors(hip, steps, 1e-8)
+    print(f"{steps} steps: relative |FD - adjoint| on 8 directions:", errs)
+    assert np.median(errs) <= median_bar and max(errs) <= max_bar, errs
+
+
+@pytest.mark.slow
+def test_config5_finite_differences(hip):
+    """1024^2, 2000 steps, gauge-misfit loss: the adjoint's directional derivatives against
+    central differences of the GPU forward on 8 smooth random directions in (eta0, b).  The
+    discrete map is only piecewise smooth (minmod, wave-speed and branch kinks everywhere),
+    so agreement is limited by kink crossings; the bound here is what that allows."""
+    errs = config5_fd_errors(hip, 2000, 1e-8)
     print("relative |FD - adjoint| on 8 directions:", errs)
     assert np.median(errs) < 1e-4 and max(errs) < 1e-3, errs
 

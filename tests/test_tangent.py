@@ -160,12 +160,14 @@ def test_schedule_is_optimal():
 
 def test_record_plan_is_the_best_that_fits():
     """The budget plan (the last K steps' stage records, a checkpoint every m steps before
-    them) maximises K within the budget: brute force over every (K, m) on small cases, in
-    units of one array (n_total = 1 cell, 8 bytes)."""
+    them, two segment buffers of m records when there are two segments or more) maximises
+    K within the budget: brute force over every (K, m) on small cases, in units of one
+    array (n_total = 1 cell, 8 bytes)."""
     from swe_hip import adjoint_plan_budget
 
     def need(n, K, m):
-        return 9 * max(K, m) + 3 * (-(-(n - K) // m))
+        nck = -(-(n - K) // m)
+        return 9 * max(K, 2 * m if nck >= 2 else m) + 3 * nck
 
     for n in range(1, 40):
         for arrays in list(range(3, 12 * n, 7)) + [9 * n, 9 * n + 5]:
