@@ -92,6 +92,16 @@ const char *swe_hip_error_string(int code);
  * of the calling thread's last failed swe_hip_create (which leaves no context behind). */
 const char *swe_hip_last_error(const swe_hip_ctx *ctx);
 
+/* Outcome of the last stepping call on ctx (swe_hip_step, swe_hip_step_adaptive[_noalloc],
+ * swe_hip_time_steps): *step = steps it started, *stage = the stage (1..3) whose output was
+ * non-finite, or 0.  When *stage != 0 the call returned SWE_ERR_NONFINITE at step *step
+ * (1-based within the call) and did nothing after that stage, as the reference's
+ * rk3_step throws there (omp_kernels.hpp:322-330, omp_solver.hpp:250-338): the state is
+ * U^n of that step when stage 1 or 2 failed, the positivity-floored stage-3 output when
+ * stage 3 did.  Replaces what that exception ends (the run loop, omp_solver.hpp:185-214).
+ * Either output may be NULL. */
+int swe_hip_last_status(const swe_hip_ctx *ctx, int64_t *step, int *stage);
+
 /* Number of visible HIP devices (0 when there is no GPU). */
 int swe_hip_device_count(void);
 
@@ -124,13 +134,15 @@ int swe_hip_stable_dt(swe_hip_ctx *ctx, double cfl, double *dt_out, double *max_
 
 /* Advance nsteps SSP-RK3 steps with a fixed dt: the reference's run loop
  * (omp_solver.hpp:185-214 / rk3_step :250-338; cuda_solver.hpp:195-227 / :400-417).
- * Non-finite states are detected per stage on the device and reported once per call
- * (SWE_ERR_NONFINITE) instead of at the first bad stage. */
+ * A non-finite stage output is detected on the device and stops the call there, as the
+ * reference's throw does: the later launches of the call do nothing, the call returns
+ * SWE_ERR_NONFINITE and swe_hip_last_status names the step and stage. */
 int swe_hip_step(swe_hip_ctx *ctx, double dt, int64_t nsteps);
 
 /* Advance nsteps steps, each with dt = compute_stable_dt(U^n, cfl) evaluated on the device
  * (extension: the reference fixes dt = end_time/steps, omp_solver.hpp:99).  dt_hist (may be
- * NULL) receives the nsteps dt values used. */
+ * NULL) receives the nsteps dt values used (quiet NaN after a non-finite stop, whose step
+ * and stage swe_hip_last_status gives). */
 int swe_hip_step_adaptive(swe_hip_ctx *ctx, double cfl, int64_t nsteps, double *dt_hist);
 
 /* swe_hip_step_adaptive that allocates nothing: no counterpart in the reference; for callers
@@ -215,7 +227,10 @@ int swe_hip_batch_step(swe_hip_batch *batch, int64_t nsteps);
  * segmented max reduction on the device); dt_hist (nsteps x count, row-major) may be NULL. */
 int swe_hip_batch_step_adaptive(swe_hip_batch *batch, double cfl, int64_t nsteps,
                                 double *dt_hist);
-/* Per-instance non-finite flags of the last step call (count ints). */
+/* Per-instance outcome of the last step call (count ints): 0, or the stage (1..3) whose
+ * output was non-finite -- that instance stopped there as a single grid does (its later
+ * launches did nothing; its dt_hist entries after that are quiet NaN) while the others
+ * went on. */
 int swe_hip_batch_nonfinite(swe_hip_batch *batch, int32_t *flags);
 
 /* Launch shape: interior rows per tile and persistent waves per CU (0 = one tile per wave).

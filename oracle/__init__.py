@@ -112,7 +112,7 @@ def lib():
         L.oracle_mod_riemann.argtypes = [C.c_double, C.c_int, C.c_int, _dp, _dp, _dp,
                                          C.POINTER(C.c_int)]
         L.oracle_run.argtypes = [C.POINTER(OracleGrid), _dp, _dp, _dp, _dp, C.c_int, C.c_double,
-                                 C.c_long, _dp, C.POINTER(C.c_long)]
+                                 C.c_long, _dp, C.POINTER(C.c_long), C.POINTER(C.c_int)]
         L.oracle_num_threads.restype = C.c_int
         _lib = L
     return _lib
@@ -251,19 +251,30 @@ def enforce_positivity(gr: OracleGrid, h, hu, hv) -> int:
     return lib().oracle_enforce_positivity(C.byref(gr), _p(h), _p(hu), _p(hv))
 
 
+def run_status(gr: OracleGrid, h, hu, hv, B, nsteps, dt=None, cfl=None):
+    """Advance in place.  Exactly one of dt (fixed) or cfl (adaptive per step).
+
+    Returns (rc, steps_done, dt_hist, failed_stage): on a non-finite state (rc 1) the run
+    stopped in stage failed_stage (1..3) of step steps_done + 1, and (h, hu, hv) are left as
+    the reference's throw leaves them (omp_solver.hpp:250-338)."""
+    assert (dt is None) != (cfl is None)
+    hist = np.zeros(max(nsteps, 1))
+    done = C.c_long()
+    stage = C.c_int()
+    mode, val = (0, dt) if dt is not None else (1, cfl)
+    rc = lib().oracle_run(C.byref(gr), _p(h), _p(hu), _p(hv), _p(B), mode, val, nsteps,
+                          _p(hist), C.byref(done), C.byref(stage))
+    if rc < 0:
+        raise MemoryError("oracle_run allocation failed")
+    return rc, done.value, hist[:nsteps], stage.value
+
+
 def run(gr: OracleGrid, h, hu, hv, B, nsteps, dt=None, cfl=None):
     """Advance in place.  Exactly one of dt (fixed) or cfl (adaptive per step).
 
     Returns (rc, steps_done, dt_hist)."""
-    assert (dt is None) != (cfl is None)
-    hist = np.zeros(max(nsteps, 1))
-    done = C.c_long()
-    mode, val = (0, dt) if dt is not None else (1, cfl)
-    rc = lib().oracle_run(C.byref(gr), _p(h), _p(hu), _p(hv), _p(B), mode, val, nsteps,
-                          _p(hist), C.byref(done))
-    if rc < 0:
-        raise MemoryError("oracle_run allocation failed")
-    return rc, done.value, hist[:nsteps]
+    rc, done, hist, _ = run_status(gr, h, hu, hv, B, nsteps, dt=dt, cfl=cfl)
+    return rc, done, hist
 
 
 # ---------------------------------------------------------------- modular serial solver

@@ -369,7 +369,9 @@ static void compute_rhs(oracle_ws *w, const double *h, const double *hu, const d
     oracle_divergence(&w->gr, w->f, w->rhs[0], w->rhs[1], w->rhs[2]);
 }
 
-/* Returns 0, or 1 on a non-finite state (the reference throws at the first stage). */
+/* Returns 0, or the stage (1..3) whose output was non-finite: the reference throws there
+ * (omp_kernels.hpp:322-330), leaving U^n untouched after a stage-1 or stage-2 failure and the
+ * floored stage-3 output in U after a stage-3 failure (omp_solver.hpp:250-338). */
 static int rk3_step(oracle_ws *w, double *h, double *hu, double *hv, double dt) {
     const long n = w->n;
     double *r0 = w->rhs[0], *r1 = w->rhs[1], *r2 = w->rhs[2];
@@ -383,7 +385,7 @@ static int rk3_step(oracle_ws *w, double *h, double *hu, double *hv, double dt) 
         w->hv1[k] = hv[k] + dt * r2[k];
     }
     bad |= oracle_enforce_positivity(&w->gr, w->h1, w->hu1, w->hv1);
-    if (bad) return 1;
+    if (bad) return 1;  /* stage 1 */
     oracle_apply_bc(&w->gr, w->h1, w->hu1, w->hv1);
     compute_rhs(w, w->h1, w->hu1, w->hv1);
 #pragma omp parallel for schedule(static)
@@ -393,7 +395,7 @@ static int rk3_step(oracle_ws *w, double *h, double *hu, double *hv, double dt) 
         w->hv2[k] = 0.75 * hv[k] + 0.25 * (w->hv1[k] + dt * r2[k]);
     }
     bad |= oracle_enforce_positivity(&w->gr, w->h2, w->hu2, w->hv2);
-    if (bad) return 1;
+    if (bad) return 2;
     oracle_apply_bc(&w->gr, w->h2, w->hu2, w->hv2);
     compute_rhs(w, w->h2, w->hu2, w->hv2);
 #pragma omp parallel for schedule(static)
@@ -403,7 +405,7 @@ static int rk3_step(oracle_ws *w, double *h, double *hu, double *hv, double dt) 
         hv[k] = (1.0 / 3.0) * hv[k] + (2.0 / 3.0) * (w->hv2[k] + dt * r2[k]);
     }
     bad |= oracle_enforce_positivity(&w->gr, h, hu, hv);
-    if (bad) return 1;
+    if (bad) return 3;
     oracle_apply_bc(&w->gr, h, hu, hv);
     return 0;
 }
@@ -429,10 +431,13 @@ int oracle_rhs(const oracle_grid *gr, double *h, double *hu, double *hv, const d
  *   dt_mode 0: fixed dt = dt_or_cfl   (reference behaviour, omp_solver.hpp:99)
  *   dt_mode 1: adaptive, dt = compute_stable_dt(U^n, cfl = dt_or_cfl) before each step
  * dt_hist (optional, nsteps entries) receives the dt used by each step.
- * Returns 0, 1 on non-finite state (steps_done reports progress), -1 on allocation failure.
+ * Returns 0, 1 on non-finite state (steps_done reports progress: the steps completed before
+ * the failing one; failed_stage, optional, the stage 1..3 that failed, else 0), -1 on
+ * allocation failure.
  */
 int oracle_run(const oracle_grid *gr, double *h, double *hu, double *hv, const double *B,
-               int dt_mode, double dt_or_cfl, long nsteps, double *dt_hist, long *steps_done) {
+               int dt_mode, double dt_or_cfl, long nsteps, double *dt_hist, long *steps_done,
+               int *failed_stage) {
     oracle_ws w;
     if (ws_init(&w, gr, B)) return -1;
     int rc = 0;
@@ -441,9 +446,15 @@ int oracle_run(const oracle_grid *gr, double *h, double *hu, double *hv, const d
         double dt = dt_or_cfl;
         if (dt_mode == 1) dt = oracle_stable_dt(gr, h, hu, hv, dt_or_cfl, NULL);
         if (dt_hist) dt_hist[s] = dt;
-        if (rk3_step(&w, h, hu, hv, dt)) { rc = 1; break; }
+        const int st = rk3_step(&w, h, hu, hv, dt);
+        if (st) {
+            if (failed_stage) *failed_stage = st;
+            rc = 1;
+            break;
+        }
     }
     if (steps_done) *steps_done = s;
+    if (failed_stage && rc == 0) *failed_stage = 0;
     ws_free(&w);
     return rc;
 }

@@ -267,7 +267,8 @@ int forward_step_via(swe_hip_adj *a, const State &in, const State &u1, const Sta
         ADJ_CHECK(a, swe::launch_speed_reduce(c->G, in.p[0], in.p[1], in.p[2], a->partial, 1024,
                                               c->stream));
         ADJ_CHECK(a, swe::launch_finalize_dt(a->partial, 1024, cfl, c->G.dx, c->G.dy, c->scal,
-                                             a->dts, a->counter, a->nsteps, c->stream));
+                                             a->dts, a->counter, a->nsteps, c->flag,
+                                             c->stream));
     }
     const int rc = stages_12(a, in, u1, u2, p);
     if (rc) return rc;
@@ -478,12 +479,17 @@ int backward_records(swe_hip_adj *a) {
     return SWE_OK;
 }
 
+// The forward's run status words (swe::kStatusWords): a non-finite stage stopped it there.
 int check_nonfinite(swe_hip_adj *a) {
-    int flag = 0;
-    ADJ_CHECK(a, hipMemcpyAsync(&flag, a->ctx->flag, sizeof(int), hipMemcpyDeviceToHost,
+    int w[swe::kStatusWords] = {};
+    ADJ_CHECK(a, hipMemcpyAsync(w, a->ctx->flag, sizeof(w), hipMemcpyDeviceToHost,
                                 a->ctx->stream));
     ADJ_CHECK(a, hipStreamSynchronize(a->ctx->stream));
-    if (flag) return afail(a, SWE_ERR_NONFINITE, "non-finite state in the forward pass");
+    for (int s = 1; s <= 3; ++s)
+        if (w[s] & swe::kStatusBad)
+            return afail(a, SWE_ERR_NONFINITE,
+                         "non-finite state in the forward pass at stage " + std::to_string(s) +
+                             " of step " + std::to_string(w[0]));
     return SWE_OK;
 }
 
@@ -774,7 +780,7 @@ int swe_hip_adj_forward(swe_hip_adj *a, int32_t dt_mode, double value, double *d
         cfl = value;
         ADJ_CHECK(a, swe::launch_fill_int(reinterpret_cast<int *>(a->counter), 0, 2, c->stream));
     }
-    ADJ_CHECK(a, swe::launch_fill_int(c->flag, 0, 1, c->stream));
+    ADJ_CHECK(a, swe::launch_fill_int(c->flag, 0, swe::kStatusWords, c->stream));
     int rc = SWE_OK;
     if (a->records) {
         rc = forward_records(a, cfl);

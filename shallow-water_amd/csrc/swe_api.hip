@@ -116,7 +116,7 @@ int enqueue_step(swe_hip_ctx *c, int mode, double cfl) {
         if (speed && stage == 3) {
             SWE_CHECK(c, swe::launch_finalize_dt(c->partial, (int)(grid.x * grid.y), cfl, c->G.dx,
                                                  c->G.dy, c->scal, c->hist, c->counter,
-                                                 c->hist_cap, c->stream));
+                                                 c->hist_cap, c->flag, c->stream));
         }
     }
     return SWE_OK;
@@ -185,12 +185,25 @@ int scratch_for_device(swe_hip_ctx *c, double **out) {
     return SWE_OK;
 }
 
+// Reads the run status words of the call that just ended.  A non-finite stage stopped the
+// call there (the kernels' first-bad-stage stop): the error names the step (1-based,
+// counted from this call's first) and the stage, and the arrays are as that stage left them.
 int check_flag(swe_hip_ctx *c) {
-    int flag = 0;
-    SWE_CHECK(c, hipMemcpyAsync(&flag, c->flag, sizeof(int), hipMemcpyDeviceToHost, c->stream));
+    int w[swe::kStatusWords] = {};
+    SWE_CHECK(c, hipMemcpyAsync(w, c->flag, sizeof(w), hipMemcpyDeviceToHost, c->stream));
     SWE_CHECK(c, hipStreamSynchronize(c->stream));
-    if (flag) return fail(c, SWE_ERR_NONFINITE, "non-finite state detected in HIP solver");
-    return SWE_OK;
+    c->last_status_step = w[0];
+    c->last_status_stage = 0;
+    for (int s = 1; s <= 3; ++s)
+        if (w[s] & swe::kStatusBad) c->last_status_stage = s;
+    if (c->last_status_stage == 0) return SWE_OK;
+    if (w[0] == 0)  // a right-hand-side evaluation alone
+        return fail(c, SWE_ERR_NONFINITE,
+                    "non-finite state detected in HIP solver (right-hand side)");
+    return fail(c, SWE_ERR_NONFINITE,
+                "non-finite state detected in HIP solver at stage " +
+                    std::to_string(c->last_status_stage) + " of step " + std::to_string(w[0]) +
+                    " of this call");
 }
 
 int upload_common(swe_hip_ctx *c, const double *h, const double *hu, const double *hv,
@@ -228,6 +241,13 @@ const char *swe_hip_error_string(int code) {
 
 const char *swe_hip_last_error(const swe_hip_ctx *ctx) {
     return ctx ? ctx->err.c_str() : g_create_err.c_str();
+}
+
+int swe_hip_last_status(const swe_hip_ctx *c, int64_t *step, int *stage) {
+    if (!c) return SWE_ERR_INVALID;
+    if (step) *step = c->last_status_step;
+    if (stage) *stage = c->last_status_stage;
+    return SWE_OK;
 }
 
 int swe_hip_device_count(void) {
@@ -290,13 +310,14 @@ int swe_hip_create(const swe_grid_desc *d, int device, swe_hip_ctx **out) {
     for (double **a : arrays)
         if (hipMalloc(a, bytes) != hipSuccess) return bad(SWE_ERR_OOM, "hipMalloc");
     if (hipMalloc(&c->scal, 4 * sizeof(double)) != hipSuccess) return bad(SWE_ERR_OOM, "scal");
-    if (hipMalloc(&c->flag, sizeof(int)) != hipSuccess) return bad(SWE_ERR_OOM, "flag");
+    if (hipMalloc(&c->flag, sizeof(int) * swe::kStatusWords) != hipSuccess)
+        return bad(SWE_ERR_OOM, "status words");
     if (hipMalloc(&c->counter, sizeof(long long)) != hipSuccess) return bad(SWE_ERR_OOM, "ctr");
     for (double **a : arrays)
         if (swe::launch_fill_double(*a, 0.0, (long)c->n_total, c->stream) != hipSuccess)
             return bad(SWE_ERR_HIP, "fill");
     if (swe::launch_fill_double(c->scal, 0.0, 4, c->stream) != hipSuccess ||
-        swe::launch_fill_int(c->flag, 0, 1, c->stream) != hipSuccess)
+        swe::launch_fill_int(c->flag, 0, swe::kStatusWords, c->stream) != hipSuccess)
         return bad(SWE_ERR_HIP, "fill");
     if (hipStreamSynchronize(c->stream) != hipSuccess) return bad(SWE_ERR_HIP, "sync");
     *out = c;
@@ -369,7 +390,7 @@ int swe_hip_stable_dt(swe_hip_ctx *c, double cfl, double *dt_out, double *ms_out
     SWE_CHECK(c, swe::launch_speed_reduce(c->G, c->U[0], c->U[1], c->U[2], c->partial, blocks,
                                           c->stream));
     SWE_CHECK(c, swe::launch_finalize_dt(c->partial, blocks, cfl, c->G.dx, c->G.dy, c->scal,
-                                         nullptr, nullptr, 0, c->stream));
+                                         nullptr, nullptr, 0, nullptr, c->stream));
     double v[2];
     SWE_CHECK(c, hipMemcpyAsync(v, c->scal, sizeof(v), hipMemcpyDeviceToHost, c->stream));
     SWE_CHECK(c, hipStreamSynchronize(c->stream));
@@ -382,7 +403,7 @@ int swe_hip_step(swe_hip_ctx *c, double dt, int64_t nsteps) {
     if (!c || nsteps < 0 || !(dt > 0.0)) return SWE_ERR_INVALID;
     SWE_CHECK(c, hipSetDevice(c->device));
     SWE_CHECK(c, swe::launch_set_scalar(c->scal, dt, c->stream));
-    SWE_CHECK(c, swe::launch_fill_int(c->flag, 0, 1, c->stream));
+    SWE_CHECK(c, swe::launch_fill_int(c->flag, 0, swe::kStatusWords, c->stream));
     const int rc = run_steps(c, 0, 0.0, nsteps);
     if (rc) return rc;
     return check_flag(c);
@@ -405,12 +426,12 @@ int swe_hip_step_adaptive(swe_hip_ctx *c, double cfl, int64_t nsteps, double *dt
         c->hist_cap = nsteps + 1;
     }
     SWE_CHECK(c, swe::launch_fill_int(reinterpret_cast<int *>(c->counter), 0, 2, c->stream));
-    SWE_CHECK(c, swe::launch_fill_int(c->flag, 0, 1, c->stream));
+    SWE_CHECK(c, swe::launch_fill_int(c->flag, 0, swe::kStatusWords, c->stream));
     // dt of the first step from U^0
     SWE_CHECK(c, swe::launch_speed_reduce(c->G, c->U[0], c->U[1], c->U[2], c->partial, 1024,
                                           c->stream));
     SWE_CHECK(c, swe::launch_finalize_dt(c->partial, 1024, cfl, c->G.dx, c->G.dy, c->scal,
-                                         c->hist, c->counter, c->hist_cap, c->stream));
+                                         c->hist, c->counter, c->hist_cap, c->flag, c->stream));
     rc = run_steps(c, 1, cfl, nsteps);
     if (rc) return rc;
     if (dt_hist && nsteps > 0)
@@ -441,12 +462,12 @@ int swe_hip_step_adaptive_noalloc(swe_hip_ctx *c, double cfl, int64_t nsteps,
     const long long hist_cap = dt_hist_device ? nsteps : 0;
     // the counter and the flag start at 0 (fill kernels, no memset node in a capture)
     SWE_CHECK(c, swe::launch_fill_int(reinterpret_cast<int *>(counter), 0, 2, c->stream));
-    SWE_CHECK(c, swe::launch_fill_int(c->flag, 0, 1, c->stream));
+    SWE_CHECK(c, swe::launch_fill_int(c->flag, 0, swe::kStatusWords, c->stream));
     // dt of the first step from U^0
     SWE_CHECK(c, swe::launch_speed_reduce(c->G, c->U[0], c->U[1], c->U[2], partial, 1024,
                                           c->stream));
     SWE_CHECK(c, swe::launch_finalize_dt(partial, 1024, cfl, c->G.dx, c->G.dy, c->scal,
-                                         dt_hist_device, counter, hist_cap, c->stream));
+                                         dt_hist_device, counter, hist_cap, c->flag, c->stream));
     for (int64_t s = 0; s < nsteps; ++s) {
         for (int stage = 1; stage <= 3; ++stage) {
             StageArgs a = stage_args(c, stage);
@@ -456,7 +477,7 @@ int swe_hip_step_adaptive_noalloc(swe_hip_ctx *c, double cfl, int64_t nsteps,
             if (stage == 3)
                 SWE_CHECK(c, swe::launch_finalize_dt(partial, (int)(grid.x * grid.y), cfl,
                                                      c->G.dx, c->G.dy, c->scal, dt_hist_device,
-                                                     counter, hist_cap, c->stream));
+                                                     counter, hist_cap, c->flag, c->stream));
         }
     }
     return check_flag(c);
@@ -543,7 +564,7 @@ int swe_hip_time_steps(swe_hip_ctx *c, double dt, int64_t nsteps, double *stage_
     if (!c || nsteps < 1 || !(dt > 0.0)) return SWE_ERR_INVALID;
     SWE_CHECK(c, hipSetDevice(c->device));
     SWE_CHECK(c, swe::launch_set_scalar(c->scal, dt, c->stream));
-    SWE_CHECK(c, swe::launch_fill_int(c->flag, 0, 1, c->stream));
+    SWE_CHECK(c, swe::launch_fill_int(c->flag, 0, swe::kStatusWords, c->stream));
     swe::EventSet ev((size_t)nsteps * 3 + 1);
     SWE_CHECK(c, ev.status);
     int rc = SWE_OK;

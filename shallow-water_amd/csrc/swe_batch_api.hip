@@ -155,7 +155,8 @@ int enqueue_batch_step(swe_hip_batch *b, int mode, double cfl) {
     }
     if (adaptive)
         SWE_BCHECK(b, swe::launch_batch_finalize(b->inst_d, b->count, b->max_bits, cfl, b->dt_d,
-                                                 b->hist, b->counter, b->hist_rows, b->stream));
+                                                 b->hist, b->counter, b->hist_rows, b->flags,
+                                                 b->stream));
     return SWE_OK;
 }
 
@@ -188,15 +189,34 @@ int run_batch_steps(swe_hip_batch *b, int mode, double cfl, int64_t nsteps) {
     return SWE_OK;
 }
 
-int check_flags(swe_hip_batch *b) {
-    std::vector<int> f((size_t)b->count);
-    SWE_BCHECK(b, hipMemcpyAsync(f.data(), b->flags, sizeof(int) * f.size(), hipMemcpyDeviceToHost,
-                                 b->stream));
+// Per instance: 0, or the stage whose output was non-finite (the instance stopped there),
+// from the run status words (swe::kStatusWords per instance).
+int read_outcomes(swe_hip_batch *b, std::vector<int> &stage, std::vector<int> &step) {
+    std::vector<int> w((size_t)b->count * swe::kStatusWords);
+    SWE_BCHECK(b, hipMemcpyAsync(w.data(), b->flags, sizeof(int) * w.size(),
+                                 hipMemcpyDeviceToHost, b->stream));
     SWE_BCHECK(b, hipStreamSynchronize(b->stream));
+    stage.assign((size_t)b->count, 0);
+    step.assign((size_t)b->count, 0);
+    for (size_t i = 0; i < (size_t)b->count; ++i) {
+        const int *st = &w[i * swe::kStatusWords];
+        step[i] = st[0];
+        for (int s = 1; s <= 3; ++s)
+            if (st[s] & swe::kStatusBad) stage[i] = s;
+    }
+    return SWE_OK;
+}
+
+int check_flags(swe_hip_batch *b) {
+    std::vector<int> stage, step;
+    const int rc = read_outcomes(b, stage, step);
+    if (rc) return rc;
     for (int i = 0; i < b->count; ++i)
-        if (f[(size_t)i])
+        if (stage[(size_t)i])
             return bfail(b, SWE_ERR_NONFINITE,
-                         "non-finite state detected in ensemble instance " + std::to_string(i));
+                         "non-finite state detected in ensemble instance " + std::to_string(i) +
+                             " at stage " + std::to_string(stage[(size_t)i]) + " of step " +
+                             std::to_string(step[(size_t)i]) + " of this call");
     return SWE_OK;
 }
 
@@ -219,6 +239,7 @@ int ensure_hist(swe_hip_batch *b, long long rows) {
 
 // Per-instance max wave speed of U -> dt (history row when counting).
 int enqueue_initial_dt(swe_hip_batch *b, double cfl, bool record) {
+    // record: the start of an adaptive step call, whose status words were just cleared
     SWE_BCHECK(b, swe::launch_fill_int(reinterpret_cast<int *>(b->max_bits), 0, 2 * b->count,
                                        b->stream));
     const BatchArgs a = batch_args(b, 1);
@@ -226,7 +247,7 @@ int enqueue_initial_dt(swe_hip_batch *b, double cfl, bool record) {
     SWE_BCHECK(b, swe::launch_batch_finalize(b->inst_d, b->count, b->max_bits, cfl, b->dt_d,
                                              record ? b->hist : nullptr,
                                              record ? b->counter : nullptr, b->hist_rows,
-                                             b->stream));
+                                             record ? b->flags : nullptr, b->stream));
     return SWE_OK;
 }
 
@@ -296,14 +317,14 @@ int swe_hip_batch_create(const swe_grid_desc *descs, int32_t count, int device,
     if (hipMalloc(&b->inst_d, sizeof(BatchInst) * (size_t)count) != hipSuccess ||
         hipMalloc(&b->dt_d, sizeof(double) * (size_t)count) != hipSuccess ||
         hipMalloc(&b->max_bits, sizeof(unsigned long long) * (size_t)count) != hipSuccess ||
-        hipMalloc(&b->flags, sizeof(int) * (size_t)count) != hipSuccess ||
+        hipMalloc(&b->flags, sizeof(int) * swe::kStatusWords * (size_t)count) != hipSuccess ||
         hipMalloc(&b->counter, sizeof(long long)) != hipSuccess)
         return bad(SWE_ERR_OOM);
     if (hipMemcpyAsync(b->inst_d, b->inst_h.data(), sizeof(BatchInst) * (size_t)count,
                        hipMemcpyHostToDevice, b->stream) != hipSuccess)
         return bad(SWE_ERR_HIP);
     if (swe::launch_fill_double(b->dt_d, 0.0, count, b->stream) != hipSuccess ||
-        swe::launch_fill_int(b->flags, 0, count, b->stream) != hipSuccess ||
+        swe::launch_fill_int(b->flags, 0, count * swe::kStatusWords, b->stream) != hipSuccess ||
         swe::launch_fill_int(reinterpret_cast<int *>(b->max_bits), 0, 2 * count, b->stream) !=
             hipSuccess ||
         swe::launch_fill_int(reinterpret_cast<int *>(b->counter), 0, 2, b->stream) != hipSuccess)
@@ -398,7 +419,7 @@ int swe_hip_batch_stable_dt(swe_hip_batch *b, double cfl, double *dt_out) {
 int swe_hip_batch_step(swe_hip_batch *b, int64_t nsteps) {
     if (!b || nsteps < 0) return SWE_ERR_INVALID;
     SWE_BCHECK(b, hipSetDevice(b->device));
-    SWE_BCHECK(b, swe::launch_fill_int(b->flags, 0, b->count, b->stream));
+    SWE_BCHECK(b, swe::launch_fill_int(b->flags, 0, b->count * swe::kStatusWords, b->stream));
     const int rc = run_batch_steps(b, 0, 0.0, nsteps);
     if (rc) return rc;
     return check_flags(b);
@@ -409,7 +430,7 @@ int swe_hip_batch_step_adaptive(swe_hip_batch *b, double cfl, int64_t nsteps, do
     SWE_BCHECK(b, hipSetDevice(b->device));
     int rc = ensure_hist(b, nsteps + 1);
     if (rc) return rc;
-    SWE_BCHECK(b, swe::launch_fill_int(b->flags, 0, b->count, b->stream));
+    SWE_BCHECK(b, swe::launch_fill_int(b->flags, 0, b->count * swe::kStatusWords, b->stream));
     SWE_BCHECK(b, swe::launch_fill_int(reinterpret_cast<int *>(b->counter), 0, 2, b->stream));
     rc = enqueue_initial_dt(b, cfl, true);
     if (rc) return rc;
@@ -424,11 +445,10 @@ int swe_hip_batch_step_adaptive(swe_hip_batch *b, double cfl, int64_t nsteps, do
 
 int swe_hip_batch_nonfinite(swe_hip_batch *b, int32_t *flags) {
     if (!b || !flags) return SWE_ERR_INVALID;
-    std::vector<int> f((size_t)b->count);
-    SWE_BCHECK(b, hipMemcpyAsync(f.data(), b->flags, sizeof(int) * f.size(), hipMemcpyDeviceToHost,
-                                 b->stream));
-    SWE_BCHECK(b, hipStreamSynchronize(b->stream));
-    for (int i = 0; i < b->count; ++i) flags[i] = f[(size_t)i];
+    std::vector<int> stage, step;
+    const int rc = read_outcomes(b, stage, step);
+    if (rc) return rc;
+    for (int i = 0; i < b->count; ++i) flags[i] = stage[(size_t)i];
     return SWE_OK;
 }
 
@@ -475,6 +495,8 @@ int swe_hip_batch_autotune(swe_hip_batch *b, int *tile_rows_out, int *waves_per_
     swe::EventSet ev(2);
     SWE_BCHECK(b, ev.status);
     hipEvent_t e0 = ev[0], e1 = ev[1];
+    // every instance's stages run (no first-bad-stage stop left over from an earlier call)
+    SWE_BCHECK(b, swe::launch_fill_int(b->flags, 0, b->count * swe::kStatusWords, b->stream));
     float best = 1e30f;
     int best_t = b->tile_rows, best_w = b->waves_per_cu;
     for (int t : tiles_c) {

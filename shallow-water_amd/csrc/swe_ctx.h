@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <cstdint>
 #include <string>
 
 #include "../../include/swe_hip.h"
@@ -23,7 +24,9 @@ struct swe_hip_ctx {
     double *scal = nullptr;      // [0] dt, [1] max speed
     double *partial = nullptr;   // per-block maxima
     int partial_cap = 0;
-    int *flag = nullptr;
+    int *flag = nullptr;         // run status words (swe::kStatusWords)
+    int64_t last_status_step = 0;  // of the last stepping call: steps it started
+    int last_status_stage = 0;     // stage whose output was non-finite, 0 if none
     long long *counter = nullptr;
     double *hist = nullptr;
     long long hist_cap = 0;

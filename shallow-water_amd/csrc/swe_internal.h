@@ -50,7 +50,7 @@ struct StageArgs {
     double *out_h, *out_hu, *out_hv;
     const double *dt;   // device scalar
     double *partial;    // per-block max wave speed (SPEED launches), may be null otherwise
-    int *flag;          // non-finite flag (bit 0)
+    int *flag;          // run status words (kStatusWords, below)
     int tile_rows;      // interior rows per wave
     int nstrips;        // 60-column strips across y
 };
@@ -71,6 +71,35 @@ constexpr int kStageBlockWaves = SWE_STAGE_BLOCK_WAVES;
 constexpr int kBatchBlockWaves = SWE_BATCH_BLOCK_WAVES;
 
 __host__ __device__ inline int strips_for(int ny) { return (ny + kStripCols - 1) / kStripCols; }
+
+// Run status of one grid: kStatusWords ints in device memory, zeroed at the start of every
+// stepping call.
+//   [0]     steps started in this call (stage-1 launches that ran)
+//   [1..3]  stage s: kStatusBad = its output held a non-finite value, kStatusSkipped = a
+//           launch of stage s did nothing because an earlier stage had one.
+// The reference throws at the first stage whose output is non-finite (omp_kernels.hpp:
+// 322-330, from rk3_step, omp_solver.hpp:250-338), so its arrays stay as that stage left
+// them: U^n untouched when stage 1 or 2 failed, the bad stage-3 output in U when stage 3
+// did.  Here every stage launch first reads the words of the two OTHER stages and, when
+// either is set, only marks its own word and returns: the launch that finds the bad value
+// runs to its end (its own blocks never read its own word), the next stage sees it and
+// skips, the one after sees both, and so on, so every later launch of the call leaves the
+// arrays alone.
+constexpr int kStatusWords = 4;
+constexpr int kStatusBad = 1, kStatusSkipped = 2;
+
+// The words of the two other stages (uniform): non-zero when an earlier stage of this run
+// went non-finite.  Read before the launch's first store, so that the compiler can use
+// scalar loads; the launch then marks its own word (stage_skip) and writes nothing.
+template <int STAGE>
+__device__ __forceinline__ int stage_stop_words(const int *status) {
+    constexpr int a = STAGE % 3 + 1, b = (STAGE + 1) % 3 + 1;
+    return status[a] | status[b];
+}
+template <int STAGE>
+__device__ __forceinline__ void stage_skip(int *status) {
+    if (threadIdx.x == 0) atomicOr(status + STAGE, kStatusSkipped);
+}
 
 // stage: 0 = right-hand side only (out = rhs), 1..3 = SSP-RK3 stage with floor.
 // bc: 0 = reflecting walls (reference), 1 = transmissive (zero-gradient).
@@ -95,10 +124,11 @@ hipError_t launch_speed_reduce(const GridDev &G, const double *h, const double *
 
 // Reduce partial[0..n) to the max speed, then dt = compute_stable_dt expression.
 //   scal[0] = dt, scal[1] = max speed.  If hist != null and *counter < hist_cap,
-//   hist[*counter] = dt; *counter += 1.
+//   hist[*counter] = dt; *counter += 1.  When status (may be null) records a non-finite
+//   stage, scal is left alone and the history entry is a quiet NaN.
 hipError_t launch_finalize_dt(const double *partial, int n, double cfl, double dx, double dy,
                               double *scal, double *hist, long long *counter, long long hist_cap,
-                              hipStream_t stream);
+                              const int *status, hipStream_t stream);
 
 // Ghost fill of (h, hu, hv) in place (hu == null: scalar fill of h only), reproducing the
 // two-pass order of omp_solver.hpp:340-432 including the corner cells.
@@ -126,7 +156,7 @@ struct BatchArgs {
     int ntiles;
     const double *dt;             // per instance
     unsigned long long *max_bits; // per instance, bits of a non-negative double (SPEED)
-    int *flags;                   // per instance non-finite flag
+    int *flags;                   // per instance: kStatusWords run status words
     double g, half_g;
     int split_ok;
 };
@@ -138,10 +168,12 @@ hipError_t launch_batch_stage(int stage, int bc, bool speed, bool fast, const Ba
 // Max wave speed of each instance's current state (U = base.in_*) into max_bits.
 hipError_t launch_batch_speed(const BatchArgs &B, int blocks, hipStream_t stream);
 // Per instance: dt = compute_stable_dt expression of max_bits (then zeroed); history row
-// hist[*counter * count + i] when hist != null and *counter < hist_rows; counter += 1.
+// hist[*counter * count + i] when hist != null and *counter < hist_rows; counter += 1.  An
+// instance whose status words (flags, kStatusWords each) record a non-finite stage keeps
+// its dt and gets a quiet NaN history entry.
 hipError_t launch_batch_finalize(const BatchInst *inst, int count, unsigned long long *max_bits,
                                  double cfl, double *dt, double *hist, long long *counter,
-                                 long long hist_rows, hipStream_t stream);
+                                 long long hist_rows, const int *flags, hipStream_t stream);
 
 hipError_t launch_set_scalar(double *dst, double v, hipStream_t stream);
 hipError_t launch_fill_int(int *dst, int v, int n, hipStream_t stream);

@@ -282,7 +282,18 @@ __device__ __forceinline__ MSlope half_slope_of(const MCell &m, const MCell &c, 
 
 template <int STAGE, int RECON, int RIEMANN, bool BATHY>
 __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, const StageArgs A) {
+    // scalar loads ahead of the launch's first store: dt and the other stages' run status
+    // words (kStatusWords)
+    const double dt = STAGE != 0 ? *A.dt : 0.0;
     const int lane = threadIdx.x & (kWave - 1);
+    if constexpr (STAGE != 0) {
+        if (stage_stop_words<STAGE>(A.flag)) {  // an earlier stage of this run went non-finite
+            stage_skip<STAGE>(A.flag);
+            return;
+        }
+        if (STAGE == 1 && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0)
+            atomicAdd(A.flag, 1);
+    }
     const int strip = blockIdx.x * kBlockWaves + threadIdx.x / kWave;
     if (strip >= A.nstrips) return;
     const int ng = G.ng, nx = G.nx, ny = G.ny, s = G.stride;
@@ -298,7 +309,6 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
     // flux_assembly.hpp:33-34); x faces count in the interior columns only (:16-17)
     const bool yface = lane >= 1 && lane < kWave - 2 && j >= ng - 1 && j <= ng + ny - 1;
     const double g = G.g, half_g = G.half_g;
-    const double dt = STAGE != 0 ? *A.dt : 0.0;
     int bad = 0;   // HLL without dissipation speed, or a non-finite stage output
     int fbad = 0;  // per face, then masked
     // Wave-uniform, sticky: every cell finished so far by this wave passes the modular range
@@ -499,7 +509,8 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
         row(r + 3, P3, P0, P1, P2, S1, S0, F1, F0);
     }
     if (__any(bad)) {
-        if (lane == 0) atomicOr(A.flag, 1);
+        // the right-hand side alone (STAGE 0) reports as stage 1, whose first half it is
+        if (lane == 0) atomicOr(A.flag + (STAGE == 0 ? 1 : STAGE), kStatusBad);
     }
 }
 

@@ -330,6 +330,10 @@ template <int STAGE, int BC, bool SPEED, bool FAST>
 __global__ __launch_bounds__(kWave * kStageBlockWaves)
 __attribute__((amdgpu_waves_per_eu(SWE_STAGE_WAVES_PER_EU)))
 void stage_kernel(const GridDev G, const StageArgs A) {
+    // scalar loads issued first, ahead of the launch's first store: dt and the other
+    // stages' run status words (kStatusWords)
+    const double dt = STAGE != 0 ? *A.dt : 0.0;
+    const int stop = STAGE != 0 ? stage_stop_words<STAGE>(A.flag) : 0;
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
     // 1-D grid of (row segment, strip group) tiles, segment-major
@@ -341,13 +345,21 @@ void stage_kernel(const GridDev G, const StageArgs A) {
     double speed = 0.0;
     int bad = 0;
 
+    if constexpr (STAGE != 0) {
+        if (stop) {  // an earlier stage of this run went non-finite
+            stage_skip<STAGE>(A.flag);
+            return;
+        }
+        if (STAGE == 1 && blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(A.flag, 1);
+    }
+
     if (strip < A.nstrips)
-        run_tile<STAGE, BC, SPEED, FAST>(G, A, 0, STAGE != 0 ? *A.dt : 0.0, strip,
-                                         G.ng + seg * A.tile_rows, lane, speed, bad);
+        run_tile<STAGE, BC, SPEED, FAST>(G, A, 0, dt, strip, G.ng + seg * A.tile_rows, lane,
+                                         speed, bad);
 
     if (STAGE != 0) {
         if (__any(bad)) {
-            if (lane == 0) atomicOr(A.flag, 1);
+            if (lane == 0) atomicOr(A.flag + STAGE, kStatusBad);
         }
     }
     if (SPEED) {
@@ -397,6 +409,17 @@ void batch_stage_kernel(const BatchArgs B) {
     for (int t = gw; t < B.ntiles; t += nw) {
         const BatchTile T = B.tiles[t];
         const BatchInst I = B.inst[T.inst];
+        int *const status = B.flags + (size_t)kStatusWords * T.inst;
+        if constexpr (STAGE != 0) {
+            // the instance's first-bad-stage stop (kStatusWords): the words of the two other
+            // stages, which no wave of this launch writes
+            constexpr int a = STAGE % 3 + 1, b = (STAGE + 1) % 3 + 1;
+            if (__builtin_amdgcn_readfirstlane(status[a] | status[b]) != 0) {
+                if (lane == 0) atomicOr(status + STAGE, kStatusSkipped);
+                continue;
+            }
+            if (STAGE == 1 && T.strip == 0 && T.r0 == 2 && lane == 0) atomicAdd(status, 1);
+        }
         const GridDev G = inst_grid(B, I);
         double speed = 0.0;
         int bad = 0;
@@ -407,7 +430,7 @@ void batch_stage_kernel(const BatchArgs B) {
             if (lane == 0) atomicMax(&B.max_bits[T.inst], (unsigned long long)__double_as_longlong(speed));
         }
         if (STAGE != 0 && __any(bad)) {
-            if (lane == 0) atomicOr(&B.flags[T.inst], 1);
+            if (lane == 0) atomicOr(status + STAGE, kStatusBad);
         }
     }
 }
@@ -441,12 +464,15 @@ __global__ __launch_bounds__(1024) void batch_finalize_kernel(const BatchInst *i
                                                              unsigned long long *max_bits,
                                                              double cfl, double *dt,
                                                              double *hist, long long *counter,
-                                                             long long hist_rows) {
+                                                             long long hist_rows,
+                                                             const int *flags) {
     const long long c = counter ? *counter : 0;
     for (int i = threadIdx.x; i < count; i += blockDim.x) {
+        const int *st = flags != nullptr ? flags + (size_t)kStatusWords * i : nullptr;
+        const bool stopped = st != nullptr && (st[1] | st[2] | st[3]) != 0;
         const double ms = __longlong_as_double((long long)max_bits[i]);
-        const double d = dt_from_speed(ms, cfl, inst[i].dx, inst[i].dy);
-        dt[i] = d;
+        const double d = stopped ? __builtin_nan("") : dt_from_speed(ms, cfl, inst[i].dx, inst[i].dy);
+        if (!stopped) dt[i] = d;
         max_bits[i] = 0ull;
         if (hist != nullptr && c < hist_rows) hist[c * count + i] = d;
     }
@@ -500,7 +526,8 @@ __global__ __launch_bounds__(256) void speed_reduce_kernel(const GridDev G, cons
 __global__ __launch_bounds__(256) void finalize_dt_kernel(const double *partial, int n, double cfl,
                                                           double dx, double dy, double *scal,
                                                           double *hist, long long *counter,
-                                                          long long hist_cap) {
+                                                          long long hist_cap,
+                                                          const int *status) {
     double m = 0.0;
     for (int t = threadIdx.x; t < n; t += blockDim.x) m = smax(m, partial[t]);
     for (int off = kWave / 2; off > 0; off >>= 1) m = smax(m, __shfl_xor(m, off));
@@ -511,9 +538,12 @@ __global__ __launch_bounds__(256) void finalize_dt_kernel(const double *partial,
     if (threadIdx.x == 0) {
         double r = red[0];
         for (int w = 1; w < (int)(blockDim.x / kWave); ++w) r = smax(r, red[w]);
-        const double dt = dt_from_speed(r, cfl, dx, dy);
-        scal[0] = dt;
-        scal[1] = r;
+        const bool stopped = status != nullptr && (status[1] | status[2] | status[3]) != 0;
+        const double dt = stopped ? __builtin_nan("") : dt_from_speed(r, cfl, dx, dy);
+        if (!stopped) {
+            scal[0] = dt;
+            scal[1] = r;
+        }
         if (hist != nullptr) {
             const long long c = *counter;
             if (c < hist_cap) hist[c] = dt;
@@ -634,9 +664,9 @@ hipError_t launch_batch_speed(const BatchArgs &B, int blocks, hipStream_t stream
 
 hipError_t launch_batch_finalize(const BatchInst *inst, int count, unsigned long long *max_bits,
                                  double cfl, double *dt, double *hist, long long *counter,
-                                 long long hist_rows, hipStream_t stream) {
+                                 long long hist_rows, const int *flags, hipStream_t stream) {
     hipLaunchKernelGGL(batch_finalize_kernel, dim3(1), dim3(1024), 0, stream, inst, count,
-                       max_bits, cfl, dt, hist, counter, hist_rows);
+                       max_bits, cfl, dt, hist, counter, hist_rows, flags);
     return hipGetLastError();
 }
 
@@ -650,9 +680,9 @@ hipError_t launch_speed_reduce(const GridDev &G, const double *h, const double *
 
 hipError_t launch_finalize_dt(const double *partial, int n, double cfl, double dx, double dy,
                               double *scal, double *hist, long long *counter, long long hist_cap,
-                              hipStream_t stream) {
+                              const int *status, hipStream_t stream) {
     hipLaunchKernelGGL(finalize_dt_kernel, dim3(1), dim3(256), 0, stream, partial, n, cfl, dx, dy,
-                       scal, hist, counter, hist_cap);
+                       scal, hist, counter, hist_cap, status);
     return hipGetLastError();
 }
 

@@ -67,6 +67,7 @@ _SIGS = {
                                      C.POINTER(C.c_int)]),
     "swe_hip_error_string": (C.c_char_p, [C.c_int]),
     "swe_hip_last_error": (C.c_char_p, [C.c_void_p]),
+    "swe_hip_last_status": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
     "swe_hip_device_count": (C.c_int, []),
     "swe_hip_create": (C.c_int, [C.POINTER(GridDesc), C.c_int, C.POINTER(C.c_void_p)]),
     "swe_hip_destroy": (None, [C.c_void_p]),
@@ -247,8 +248,17 @@ class HipSolver:
 
     def step_adaptive(self, cfl, nsteps=1):
         hist = np.zeros(max(int(nsteps), 1))
+        # kept for callers that catch SweError: the dts up to a non-finite stop
+        self.last_dt_hist = hist[:nsteps]
         self._check(lib().swe_hip_step_adaptive(self._ctx, float(cfl), int(nsteps), _p(hist)))
         return hist[:nsteps]
+
+    def last_status(self):
+        """(steps started, failing stage or 0) of the last stepping call: after
+        SWE_ERR_NONFINITE the call stopped at that stage of that step (1-based)."""
+        step, stage = C.c_int64(), C.c_int()
+        self._check(lib().swe_hip_last_status(self._ctx, C.byref(step), C.byref(stage)))
+        return step.value, stage.value
 
     def step_adaptive_noalloc(self, cfl, nsteps=1, dt_hist=None):
         """swe_hip_step_adaptive_noalloc: no device allocation; dt_hist is None or a float64
@@ -375,6 +385,8 @@ class HipBatch:
 
     def step_adaptive(self, cfl, nsteps=1):
         hist = np.zeros((max(int(nsteps), 1), self.count))
+        # kept for callers that catch SweError: the dts up to each instance's stop
+        self.last_dt_hist = hist[:nsteps]
         self._check(lib().swe_hip_batch_step_adaptive(self._b, float(cfl), int(nsteps), _p(hist)))
         return hist[:nsteps]
 

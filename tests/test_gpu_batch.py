@@ -122,6 +122,9 @@ def test_batch_autotune_keeps_state(hip):
 
 
 def test_batch_nonfinite_flags_per_instance(hip):
+    """A non-finite instance stops at its failing stage as a single grid does (here stage 1
+    of step 1: its arrays stay as uploaded) while the others take all their steps, equal to
+    their single-grid runs."""
     cases = members(sides=[20, 30, 40])
     cases[1]["hv"][10, 10] = np.nan
     with make_batch(hip, cases) as b:
@@ -129,7 +132,48 @@ def test_batch_nonfinite_flags_per_instance(hip):
         with pytest.raises(hip.SweError) as e:
             b.step(2)
         assert e.value.code == hip.SWE_ERR_NONFINITE
+        assert "instance 1 at stage 1 of step 1 " in str(e.value)
         assert list(b.nonfinite()) == [0, 1, 0]
+        got = [b.download(i) for i in range(len(cases))]
+    I = (slice(2, -2), slice(2, -2))
+    for comp, g_ in zip(("h", "hu", "hv"), got[1]):
+        assert same(g_[I], cases[1][comp][I]), f"stopped instance {comp}"
+    for i in (0, 2):
+        want, _ = single_run(hip, cases[i], 2, dt=1e-4)
+        for comp, g_, w_ in zip(("h", "hu", "hv"), got[i], want):
+            assert same(g_, w_), f"instance {i} {comp}"
+
+
+def test_batch_adaptive_stop_per_instance(hip):
+    """Adaptive dt: the instance that goes non-finite (a rough wet/dry state at CFL 3.0,
+    stage 2 of step 15 in the oracle) stops there with the oracle's dt history up to that
+    step and quiet NaN after it, and the oracle's arrays; the other instance's history and
+    arrays equal its single-grid run."""
+    bad = W.random_state(16, 19, seed=2, wet_dry=True)
+    good = W.random_state(16, 19, seed=2, wet_dry=False)
+    cases = [good, bad]
+    with make_batch(hip, cases) as b:
+        with pytest.raises(hip.SweError) as e:
+            b.step_adaptive(3.0, 30)
+        assert e.value.code == hip.SWE_ERR_NONFINITE
+        assert "instance 1 at stage 2 of step 15 " in str(e.value)
+        assert list(b.nonfinite()) == [0, 2]
+        hist = b.last_dt_hist
+        got = [b.download(i) for i in range(2)]
+    gr = oracle.grid(16, 19, 2, bad["dx"], bad["dy"], g=bad["g"], bc=bad["bc"])
+    B = bad["B"].copy()
+    oracle.apply_bc(gr, B)
+    U = [bad[k].copy() for k in ("h", "hu", "hv")]
+    rc, done, ohist, st = oracle.run_status(gr, *U, B, 30, cfl=3.0)
+    assert (rc, done + 1, st) == (1, 15, 2)
+    assert same(hist[:15, 1], ohist[:15]) and np.isnan(hist[15:, 1]).all()
+    I = (slice(2, -2), slice(2, -2))
+    for comp, g_, w_ in zip(("h", "hu", "hv"), got[1], U):
+        assert same(g_[I], w_[I]), f"stopped instance {comp}"
+    want, whist = single_run(hip, good, 30, cfl=3.0)
+    assert same(hist[:, 0], whist)
+    for comp, g_, w_ in zip(("h", "hu", "hv"), got[0], want):
+        assert same(g_, w_), f"running instance {comp}"
 
 
 def test_batch_rejects_mixed_parameters(hip):

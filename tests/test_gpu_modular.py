@@ -157,18 +157,35 @@ def test_lake_at_rest_plain(hip, recon, riemann):
 
 
 def test_failures_are_reported(hip):
-    """The plain path has no positivity fix: with dry cells HLL's wave speeds go NaN after
-    a few steps, where the reference throws; the HIP step reports non-finite."""
+    """The plain path has no positivity fix: with dry cells HLL finds no dissipation speed
+    in the third stage's right-hand side of the first step (the modular oracle's stage-1 and
+    stage-2 right-hand sides succeed, the stage-3 one fails), where the reference throws
+    (HLL.hpp:23); the HIP step reports it at that stage and step and stops.  (The reference
+    throws before it writes U^{n+1}; the fused stage-3 kernel has written it by the time it
+    knows, so the arrays left behind differ -- the reference's app never reads them: it ends
+    the run, apps/main.cpp:23.)"""
     z = np.load(GOLD)
     nx, ny, lx, ly = z["grid"]
     gr = oracle.grid(int(nx), int(ny), lx=float(lx), ly=float(ly))
     h, hu, hv, B = (z[f"in_{k}_dry"] for k in ("h", "hu", "hv", "B"))
+    dt = float(z["stepdt_dry"])
     assert int(z["runrc_dry_0_muscl_hll"]) != 0
+    # the failing stage from the oracle's own pieces: rhs of U, U1, U2
+    U = [h.copy(), hu.copy(), hv.copy()]
+    rc, L = oracle.mod_rhs(gr, 0, RECON_MUSCL, RIEMANN_HLL, *U, B)
+    assert rc == 0
+    U1 = [1.0 * x + dt * l for x, l in zip(U, L)]
+    rc, L = oracle.mod_rhs(gr, 0, RECON_MUSCL, RIEMANN_HLL, *U1, B)
+    assert rc == 0
+    U2 = [3.0 / 4.0 * x + 1.0 / 4.0 * y + dt / 4.0 * l for x, y, l in zip(U, U1, L)]
+    rc, _ = oracle.mod_rhs(gr, 0, RECON_MUSCL, RIEMANN_HLL, *U2, B)
+    assert rc != 0
     with solver(hip, gr, 0, RECON_MUSCL, RIEMANN_HLL) as s:
         s.upload(h, hu, hv, B)
         with pytest.raises(hip.SweError) as e:
-            s.step(float(z["stepdt_dry"]), int(z["steps"]))
+            s.step(dt, int(z["steps"]))
         assert e.value.code == hip.SWE_ERR_NONFINITE
+        assert s.last_status() == (1, 3)
 
 
 def test_rejected_combinations(hip):

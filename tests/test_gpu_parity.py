@@ -12,6 +12,7 @@ import pytest
 
 import oracle
 from swe_hip import workloads as W
+from test_oracle_invariants import STOP_CASES, stop_case
 
 pytestmark = pytest.mark.gpu
 
@@ -22,10 +23,15 @@ def bits(a):
     return np.ascontiguousarray(a).view(np.uint64)
 
 
-def assert_bitwise(got, want, what=""):
+def assert_bitwise(got, want, what="", nan_any=False):
+    """Every float64 bit equal.  nan_any: a NaN matches any NaN -- IEEE 754 leaves the sign and
+    payload of the NaN an invalid operation produces to the hardware (x86 gives -NaN, the
+    GPU +NaN), so states that have gone non-finite compare NaN positions only."""
     got, want = np.asarray(got), np.asarray(want)
     assert got.shape == want.shape, what
     neq = bits(got) != bits(want)
+    if nan_any:
+        neq &= ~(np.isnan(got) & np.isnan(want))
     if neq.any():
         idx = np.argwhere(neq)
         d = np.abs(got - want)
@@ -243,6 +249,10 @@ def test_nonfinite_is_reported(hip):
         with pytest.raises(hip.SweError) as e:
             s.step(1e-4, 2)
         assert e.value.code == hip.SWE_ERR_NONFINITE
+        assert s.last_status() == (1, 1)
+        got = s.download()
+    for comp, g_ in zip(("h", "hu", "hv"), got):
+        assert_bitwise(g_[2:-2, 2:-2], c[comp][2:-2, 2:-2], f"stopped at stage 1 {comp}")
 
 
 def test_wetdry_divergence_matches_reference(hip):
@@ -261,6 +271,76 @@ def test_wetdry_divergence_matches_reference(hip):
         with pytest.raises(hip.SweError) as e:
             s.step(float(z["dt"][0]), 7)
         assert e.value.code == hip.SWE_ERR_NONFINITE
+
+
+INTERIOR = (slice(2, -2), slice(2, -2))
+
+
+@pytest.mark.parametrize("name", sorted(STOP_CASES))
+def test_first_bad_stage_stop(hip, name):
+    """A non-finite stage output stops the call there, as the reference's throw ends
+    rk3_step (omp_kernels.hpp:322-330): one 20-step call reports the oracle's failing step
+    and stage (one case per stage) and leaves the arrays as the reference does -- U^n of the
+    failing step after a stage-1 or stage-2 failure, the floored stage-3 output after a
+    stage-3 failure (interior bitwise; the download refills the ghosts).  The next call
+    starts afresh and fails in its first step: again at that stage when U^n was kept, at
+    stage 1 from the non-finite stage-3 output."""
+    c, gr, B, U, dt, stage, step = stop_case(name)
+    rc, done, _, st = oracle.run_status(gr, *U, B, 20, dt=dt)
+    assert (rc, done + 1, st) == (1, step, stage)
+    with gpu_solver(hip, c) as s:
+        with pytest.raises(hip.SweError) as e:
+            s.step(dt, 20)
+        assert e.value.code == hip.SWE_ERR_NONFINITE
+        assert f"stage {stage} of step {step} " in str(e.value)
+        assert s.last_status() == (step, stage)
+        got = s.download()
+        for comp, g_, w_ in zip(("h", "hu", "hv"), got, U):
+            assert_bitwise(g_[INTERIOR], w_[INTERIOR], f"{name} stopped state {comp}",
+                           nan_any=True)
+        with pytest.raises(hip.SweError):
+            s.step(dt, 5)
+        assert s.last_status() == (1, stage if stage != 3 else 1)
+
+
+def test_stopped_state_is_the_references(hip):
+    """The reference's own run of the wet/dry lake (48^2, 10 steps, golden fixture written
+    from its compiled kernels) throws in stage 1 of step 5; one HIP call of 10 steps stops
+    at the same stage and step with the reference's final arrays (interior bitwise)."""
+    m = json.load(open(os.path.join(GOLDEN, "manifest.json")))["steps_wetdry_48_10"]
+    z = np.load(os.path.join(GOLDEN, "steps_wetdry_48_10.npz"))
+    assert m["rc"] == 1
+    with hip.HipSolver(48, 48, m["dx"], m["dy"], hip.G_REF, 0, 2) as s:
+        s.upload(z["h"], z["hu"], z["hv"], z["B"])
+        with pytest.raises(hip.SweError) as e:
+            s.step(float(z["dt"][0]), 10)
+        assert e.value.code == hip.SWE_ERR_NONFINITE
+        assert s.last_status() == (5, 1)
+        got = s.download()
+    for comp, g_ in zip(("h", "hu", "hv"), got):
+        assert_bitwise(g_[INTERIOR], z["out_" + comp][INTERIOR], f"wetdry stop {comp}",
+                       nan_any=True)
+
+
+def test_first_bad_stage_stop_adaptive(hip):
+    """Adaptive dt (CFL 3.0 on a rough wet/dry state): the run goes non-finite in stage 2 of
+    step 15; the dt history up to that step equals the oracle's bitwise, later entries are
+    quiet NaN (no dt is computed from a stopped state), and the state is U^n of step 15."""
+    c = W.random_state(16, 19, seed=2, wet_dry=True)
+    gr, h, hu, hv, B = oracle_state(c)
+    rc, done, hist, st = oracle.run_status(gr, h, hu, hv, B, 30, cfl=3.0)
+    assert (rc, done + 1, st) == (1, 15, 2)
+    with gpu_solver(hip, c) as s:
+        with pytest.raises(hip.SweError) as e:
+            s.step_adaptive(3.0, 30)
+        assert e.value.code == hip.SWE_ERR_NONFINITE
+        assert s.last_status() == (15, 2)
+        got_hist = s.last_dt_hist
+        got = s.download()
+    assert_bitwise(got_hist[:15], hist[:15], "dt history up to the stop")
+    assert np.isnan(got_hist[15:]).all()
+    for comp, g_, w_ in zip(("h", "hu", "hv"), got, (h, hu, hv)):
+        assert_bitwise(g_[INTERIOR], w_[INTERIOR], f"adaptive stop {comp}", nan_any=True)
 
 
 def test_lake_at_rest_wet_roundoff(hip):
@@ -329,7 +409,8 @@ def test_lake_4096_wetdry_as_stated_stepwise(hip, bump):
     reference scheme produces a non-finite state (omp_kernels.hpp:309-331: at wet/dry
     fronts MUSCL on eta gives dry cells a face depth and the hydrostatic correction gives
     them momentum, so u = hu/h blows up once h is just above the 1e-8 floor).  At that same
-    step the HIP path reports SWE_ERR_NONFINITE."""
+    step the HIP path reports SWE_ERR_NONFINITE and leaves the arrays the reference leaves;
+    so does one call of 200 steps from the initial state (first-bad-stage stop)."""
     c = W.lake_wet_dry(4096, seed=2, bump=bump)
     gr, h, hu, hv, B = oracle_state(c)
     dt, _ = oracle.stable_dt(gr, h, hu, hv, 0.4)
@@ -341,6 +422,12 @@ def test_lake_4096_wetdry_as_stated_stepwise(hip, bump):
                 with pytest.raises(hip.SweError) as e:
                     s.step(dt, 1)
                 assert e.value.code == hip.SWE_ERR_NONFINITE
+                # stopped where the reference throws, with its arrays (interior)
+                got = s.download()
+                for comp, g_, w_ in zip(("h", "hu", "hv"), got, (h, hu, hv)):
+                    assert_bitwise(g_[2:-2, 2:-2], w_[2:-2, 2:-2],
+                                   f"config 2 (bump={bump}) stopped at step {step} {comp}",
+                                   nan_any=True)
                 failed_at = step
                 break
             s.step(dt, 1)
@@ -348,8 +435,18 @@ def test_lake_4096_wetdry_as_stated_stepwise(hip, bump):
             for comp, g_, w_ in zip(("h", "hu", "hv"), got, (h, hu, hv)):
                 assert_bitwise(g_, w_, f"config 2 (bump={bump}) step {step} {comp}")
     assert failed_at is not None, "the reference scheme no longer diverges within 200 steps"
+    with gpu_solver(hip, c) as s:
+        with pytest.raises(hip.SweError) as e:
+            s.step(dt, 200)
+        assert e.value.code == hip.SWE_ERR_NONFINITE
+        status = s.last_status()
+        assert status[0] == failed_at
+        got = s.download()
+    for comp, g_, w_ in zip(("h", "hu", "hv"), got, (h, hu, hv)):
+        assert_bitwise(g_[2:-2, 2:-2], w_[2:-2, 2:-2],
+                       f"config 2 (bump={bump}) one 200-step call {comp}", nan_any=True)
     print(f"config 2 (bump={bump}): bitwise for {failed_at - 1} steps, non-finite at step "
-          f"{failed_at} in both")
+          f"{failed_at} in both; one 200-step call stops there at stage {status[1]}")
 
 
 @pytest.mark.slow

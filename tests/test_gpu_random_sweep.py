@@ -54,7 +54,7 @@ def test_random_planted_state_bitwise(hip, seed):
     h, hu, hv = (c[k].copy() for k in ("h", "hu", "hv"))
     want_rhs = oracle.rhs(gr, h.copy(), hu.copy(), hv.copy(), B)
     dt, _ = oracle.stable_dt(gr, h.copy(), hu.copy(), hv.copy(), 0.3)
-    rc, _, _ = oracle.run(gr, h, hu, hv, B, 3, dt=dt)
+    rc, done, _, stage = oracle.run_status(gr, h, hu, hv, B, 3, dt=dt)
     with hip.HipSolver.from_case(c) as s:
         s.tile_rows = tile
         s.upload(c["h"], c["hu"], c["hv"], c["B"])
@@ -63,9 +63,17 @@ def test_random_planted_state_bitwise(hip, seed):
             assert np.array_equal(bits(g_), bits(w_)), (seed, c["nx"], c["ny"], tile, "rhs", comp)
         s.upload(c["h"], c["hu"], c["hv"], c["B"])
         if rc != 0:
+            # stopped where the oracle stops, with its arrays (interior)
             with pytest.raises(hip.SweError) as e:
                 s.step(dt, 3)
             assert e.value.code == hip.SWE_ERR_NONFINITE
+            assert s.last_status() == (done + 1, stage)
+            got = s.download()
+            I = (slice(c["ng"], -c["ng"]), slice(c["ng"], -c["ng"]))
+            for comp, g_, w_ in zip(("h", "hu", "hv"), got, (h, hu, hv)):
+                # a NaN matches any NaN (IEEE 754 leaves its sign and payload to the hardware)
+                neq = (bits(g_[I]) != bits(w_[I])) & ~(np.isnan(g_[I]) & np.isnan(w_[I]))
+                assert not neq.any(), (seed, tile, "stopped", comp)
             return
         s.step(dt, 3)
         got = s.download()

@@ -202,3 +202,70 @@ def test_numpy_ghost_fill_matches_oracle():
         oracle.apply_bc(gr, a)
         W.fill_ghosts(b, 2, bc)
         assert same(a, b)
+
+
+# ---- where a run stops: the failing stage and the state it leaves ----------------------
+# Cases (workload, CFL for a fixed dt from the initial state) whose run goes non-finite in
+# each of the three stages; the reference throws there and leaves U^n untouched after a
+# stage-1 or stage-2 failure, the floored stage-3 output after a stage-3 failure.
+STOP_CASES = {
+    "stage1": (lambda: W.random_state(16, 19, seed=0), 3.0, 1, 4),
+    "stage2": (lambda: W.random_state(16, 19, seed=1, wet_dry=True), 0.4, 2, 4),
+    "stage3": (lambda: W.random_state(16, 19, seed=0), 1.5, 3, 4),
+    "lake48_stage3": (lambda: W.lake_wet_dry(48, seed=1), 0.4, 3, 4),
+}
+
+
+def stop_case(name):
+    make, cfl, stage, step = STOP_CASES[name]
+    c = make()
+    gr = oracle.grid(c["nx"], c["ny"], 2, c["dx"], c["dy"], g=c["g"], bc=c["bc"])
+    B = c["B"].copy()
+    oracle.apply_bc(gr, B)
+    U = [c[k].copy() for k in ("h", "hu", "hv")]
+    dt, _ = oracle.stable_dt(gr, *U, cfl)
+    return c, gr, B, U, dt, stage, step
+
+
+def rk3_by_stage(gr, U, B, dt):
+    """One rk3_step assembled from the oracle's primitives -- rhs (after its ghost fill),
+    the stage combinations of omp_solver.hpp:262-266, :288-292, :314-320 and
+    enforce_positivity -- in numpy.  Returns the first stage whose output is non-finite
+    (U then as the reference leaves it) or 0."""
+    r = oracle.rhs(gr, *U, B)
+    U1 = [x + dt * y for x, y in zip(U, r)]
+    if oracle.enforce_positivity(gr, *U1):
+        return 1
+    r = oracle.rhs(gr, *U1, B)
+    U2 = [0.75 * x + 0.25 * (y + dt * z) for x, y, z in zip(U, U1, r)]
+    if oracle.enforce_positivity(gr, *U2):
+        return 2
+    r = oracle.rhs(gr, *U2, B)
+    for x, y, z in zip(U, U2, r):
+        x[...] = (1.0 / 3.0) * x + (2.0 / 3.0) * (y + dt * z)
+    if oracle.enforce_positivity(gr, *U):
+        return 3
+    oracle.apply_bc(gr, *U)
+    return 0
+
+
+@pytest.mark.parametrize("name", sorted(STOP_CASES))
+def test_run_stops_at_the_failing_stage(name):
+    """oracle_run's failing step and stage, and the state it leaves, against the step
+    assembled stage by stage: the stage a run reports is the one whose output first holds a
+    non-finite value, U^n stays untouched unless that stage is 3, and the arrays match
+    bitwise (the stage-3 case over the whole array: its ghosts are those of U^n)."""
+    c, gr, B, U, dt, stage, step = stop_case(name)
+    V = [x.copy() for x in U]
+    rc, done, _, st = oracle.run_status(gr, *U, B, 20, dt=dt)
+    assert (rc, done + 1, st) == (1, step, stage)
+    for k in range(1, 21):
+        before = [x.copy() for x in V]
+        s = rk3_by_stage(gr, V, B, dt)
+        if s:
+            break
+    assert (k, s) == (step, stage)
+    I = (slice(2, -2), slice(2, -2))
+    for x, y, b in zip(U, V, before):
+        assert same(x, y)
+        assert same(x[I], b[I]) == (stage != 3)
