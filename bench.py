@@ -378,6 +378,41 @@ def serial_bench(n=4096, steps=20, cpu_n=1024, cpu_steps=2):
             "schemes": rows}
 
 
+def fused_bench(case, dt, three_launch_value, rows_list=(64, 128), steps=100):
+    """The fused one-launch step (temporal blocking across the three stages) on the headline
+    workload, per tile height: Gcell-updates/s, its ratio to the three-launch step, and the
+    final state after warmup + steps against the three-launch path's (bitwise)."""
+    import swe_hip
+
+    cells = case["nx"] * case["ny"]
+    with swe_hip.HipSolver.from_case(case) as ref:
+        ref.upload(case["h"], case["hu"], case["hv"], case["B"])
+        ref.step(dt, 2 + steps)
+        want = ref.download()
+    out = {"workload": "the headline dam break, one launch per step", "steps": steps,
+           "algorithmic_bytes_per_cell_update": 56, "three_launch_gcell_per_s": three_launch_value,
+           "tiles": []}
+    for rows in rows_list:
+        with swe_hip.HipSolver.from_case(case) as s:
+            s.upload(case["h"], case["hu"], case["hv"], case["B"])
+            s.step_fusion = rows
+            s.step(dt, 2)
+            s.sync()
+            t0 = time.perf_counter()
+            s.step(dt, steps)
+            el = time.perf_counter() - t0
+            got = s.download()
+        v = cells * steps / el / 1e9
+        same = all(np.array_equal(a.view(np.uint64), b.view(np.uint64)) for a, b in zip(got, want))
+        rec = {"rows": rows, "gcell_per_s": round(v, 3), "ms_per_step": round(el / steps * 1e3, 4),
+               "vs_three_launch": round(v / three_launch_value, 4),
+               "hbm_frac_at_56B": round(v * 1e9 * 56 / (HBM_PEAK_GBS * 1e9), 4),
+               "bitwise_vs_three_launch": bool(same)}
+        log(f"[bench] fused step, {rows}-row tiles: {rec}")
+        out["tiles"].append(rec)
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -394,6 +429,9 @@ def main():
     ap.add_argument("--ensemble-base-steps", type=int, default=30)
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-serial", action="store_true")
+    ap.add_argument("--no-fused", action="store_true")
+    ap.add_argument("--fused-rows", type=str, default="64,128",
+                    help="tile heights of the fused one-launch step to time")
     ap.add_argument("--serial-n", type=int, default=4096)
     ap.add_argument("--adjoint-steps", type=int, default=2000)
     ap.add_argument("--adjoint-plans", type=str, default="",
@@ -524,6 +562,9 @@ def main():
             info["c1"] = c1_record(info["threads"])
             log(f"[bench] C1: {info['c1']}")
         out["cpu_baseline"] = info
+    if not args.no_fused and args.precision == "exact":
+        out["fused_step"] = fused_bench(case, dt, out["value"],
+                                        [int(r) for r in args.fused_rows.split(",") if r])
     if not args.no_ensemble:
         out["batched"] = ensemble_bench(args.ensemble_steps, args.ensemble_base_steps)
     if not args.no_adjoint:

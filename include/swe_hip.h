@@ -181,6 +181,18 @@ int swe_hip_get_precision(const swe_hip_ctx *ctx);
 int swe_hip_set_scheme(swe_hip_ctx *ctx, int scheme, int recon, int riemann);
 int swe_hip_get_scheme(const swe_hip_ctx *ctx, int *scheme, int *recon, int *riemann);
 
+/* One launch per step instead of three: the three SSP-RK3 stages fused (temporal
+ * blocking across the stages), `rows` interior rows per tile (>= 4, clamped to [4, nx]),
+ * 0 = off (the default).  No counterpart in the reference, whose rk3_step
+ * (omp_solver.hpp:250-338) runs the stages one after the other; results are bitwise the
+ * same.  Each wave recomputes stages 1 and 2 on a 4- and 2-cell halo of its 52-column,
+ * `rows`-row tile, so the HBM traffic falls from 216 to about 100 bytes per cell-update
+ * while the face arithmetic grows by about a fifth.  Used by swe_hip_step and
+ * swe_hip_step_adaptive with the optimized scheme (launched directly, no graph); needs
+ * ng = 2 and a second state buffer (3 arrays, allocated on the first call). */
+int swe_hip_set_step_fusion(swe_hip_ctx *ctx, int rows);
+int swe_hip_get_step_fusion(const swe_hip_ctx *ctx);
+
 /* Use a caller stream (hipStream_t passed as void*; NULL = the context's own stream). */
 int swe_hip_set_stream(swe_hip_ctx *ctx, void *stream);
 

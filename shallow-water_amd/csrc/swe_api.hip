@@ -13,6 +13,7 @@
 #include <cstring>
 #include <mutex>
 #include <string>
+#include <utility>
 #include <vector>
 
 #include "../../include/swe_hip.h"
@@ -122,7 +123,38 @@ int enqueue_step(swe_hip_ctx *c, int mode, double cfl) {
     return SWE_OK;
 }
 
+bool fused_mode(const swe_hip_ctx *c) {
+    return c->fused_rows > 0 && c->scheme == SWE_SCHEME_OPTIMIZED;
+}
+
+// Fused steps: one launch per step (swe::launch_fused_step), U -> Unext, then the host swaps
+// the two pointer sets so that U is the state the next launch reads.  Launched directly
+// (the swap would change the pointers a captured graph holds).
+int run_fused_steps(swe_hip_ctx *c, int mode, double cfl, int64_t nsteps) {
+    drop_graph(c);  // a captured three-launch step holds the U pointers
+    const int strips = swe::fused_strips_for(c->G.ny);
+    for (int64_t s = 0; s < nsteps; ++s) {
+        StageArgs a1 = stage_args(c, 1), a2 = stage_args(c, 2), a3 = stage_args(c, 3);
+        a1.tile_rows = c->fused_rows;
+        a1.nstrips = strips;
+        a3.out_h = c->Unext[0];
+        a3.out_hu = c->Unext[1];
+        a3.out_hv = c->Unext[2];
+        dim3 grid;
+        SWE_CHECK(c, swe::launch_fused_step(c->bc, mode == 1,
+                                            c->precision == SWE_PRECISION_FAST, c->G, a1, a2,
+                                            a3, (int)(s & 1), c->stream, &grid));
+        if (mode == 1)
+            SWE_CHECK(c, swe::launch_finalize_dt(c->partial, (int)grid.x, cfl, c->G.dx, c->G.dy,
+                                                 c->scal, c->hist, c->counter, c->hist_cap,
+                                                 c->flag, c->stream));
+        for (int i = 0; i < 3; ++i) std::swap(c->U[i], c->Unext[i]);
+    }
+    return SWE_OK;
+}
+
 int run_steps(swe_hip_ctx *c, int mode, double cfl, int64_t nsteps) {
+    if (fused_mode(c)) return run_fused_steps(c, mode, cfl, nsteps);
     if (!c->use_graph || nsteps < 2) {
         for (int64_t s = 0; s < nsteps; ++s) {
             const int rc = enqueue_step(c, mode, cfl);
@@ -183,6 +215,28 @@ int scratch_for_device(swe_hip_ctx *c, double **out) {
     }
     *out = static_cast<double *>(g_scratch_ptr[c->device]);
     return SWE_OK;
+}
+
+// Status words of a fused-step call of nsteps launches (fused_step_kernel): word 1 + (k & 1)
+// of launch k holds bit s-1 for a non-finite stage s.  On a failure in step k the state the
+// reference leaves is U^{k-1} (stage 1 or 2) or the stage-3 output of step k, i.e. the
+// pointer set after k-1 or k swaps; the host made nsteps, so one more swap when the
+// difference is odd.
+int check_flag_fused(swe_hip_ctx *c, int64_t nsteps) {
+    int w[swe::kStatusWords] = {};
+    SWE_CHECK(c, hipMemcpyAsync(w, c->flag, sizeof(w), hipMemcpyDeviceToHost, c->stream));
+    SWE_CHECK(c, hipStreamSynchronize(c->stream));
+    c->last_status_step = w[0];
+    const int bits = (w[1] | w[2]) & 7;
+    c->last_status_stage = bits & 1 ? 1 : (bits & 2 ? 2 : (bits & 4 ? 3 : 0));
+    if (c->last_status_stage == 0) return SWE_OK;
+    const int64_t k = w[0], m = c->last_status_stage == 3 ? k : k - 1;
+    if (((nsteps - m) & 1) != 0)
+        for (int i = 0; i < 3; ++i) std::swap(c->U[i], c->Unext[i]);
+    return fail(c, SWE_ERR_NONFINITE,
+                "non-finite state detected in HIP solver at stage " +
+                    std::to_string(c->last_status_stage) + " of step " + std::to_string(k) +
+                    " of this call");
 }
 
 // Reads the run status words of the call that just ended.  A non-finite stage stopped the
@@ -333,6 +387,7 @@ void swe_hip_destroy(swe_hip_ctx *c) {
         (void)hipFree(c->U[i]);
         (void)hipFree(c->U1[i]);
         (void)hipFree(c->U2[i]);
+        (void)hipFree(c->Unext[i]);
     }
     (void)hipFree(c->B);
     (void)hipFree(c->scal);
@@ -406,7 +461,7 @@ int swe_hip_step(swe_hip_ctx *c, double dt, int64_t nsteps) {
     SWE_CHECK(c, swe::launch_fill_int(c->flag, 0, swe::kStatusWords, c->stream));
     const int rc = run_steps(c, 0, 0.0, nsteps);
     if (rc) return rc;
-    return check_flag(c);
+    return fused_mode(c) ? check_flag_fused(c, nsteps) : check_flag(c);
 }
 
 int swe_hip_step_adaptive(swe_hip_ctx *c, double cfl, int64_t nsteps, double *dt_hist) {
@@ -415,7 +470,9 @@ int swe_hip_step_adaptive(swe_hip_ctx *c, double cfl, int64_t nsteps, double *dt
         return fail(c, SWE_ERR_INVALID, "the serial scheme family runs with a fixed dt");
     SWE_CHECK(c, hipSetDevice(c->device));
     const dim3 grid = swe::stage_grid(c->G, c->tile_rows);
-    int rc = ensure_partial(c, (int)std::max<unsigned>(1024u, grid.x * grid.y));
+    int need = (int)std::max<unsigned>(1024u, grid.x * grid.y);
+    if (fused_mode(c)) need = std::max(need, swe::fused_step_blocks(c->G, c->fused_rows));
+    int rc = ensure_partial(c, need);
     if (rc) return rc;
     if (c->hist_cap < nsteps + 1) {
         drop_graph(c);  // the graph bakes the history pointer
@@ -437,7 +494,7 @@ int swe_hip_step_adaptive(swe_hip_ctx *c, double cfl, int64_t nsteps, double *dt
     if (dt_hist && nsteps > 0)
         SWE_CHECK(c, hipMemcpyAsync(dt_hist, c->hist, sizeof(double) * (size_t)nsteps,
                                     hipMemcpyDeviceToHost, c->stream));
-    return check_flag(c);
+    return fused_mode(c) ? check_flag_fused(c, nsteps) : check_flag(c);
 }
 
 int swe_hip_step_adaptive_noalloc(swe_hip_ctx *c, double cfl, int64_t nsteps,
@@ -543,6 +600,34 @@ int swe_hip_set_scheme(swe_hip_ctx *c, int scheme, int recon, int riemann) {
     c->riemann = riemann;
     return SWE_OK;
 }
+
+int swe_hip_set_step_fusion(swe_hip_ctx *c, int rows) {
+    if (!c || rows < 0) return SWE_ERR_INVALID;
+    SWE_CHECK(c, hipSetDevice(c->device));
+    if (rows == 0) {
+        c->fused_rows = 0;
+        return SWE_OK;
+    }
+    if (c->G.ng != 2)
+        return fail(c, SWE_ERR_INVALID, "fused steps need ghost width ng = 2");
+    if (!c->Unext[0]) {
+        const size_t bytes = c->n_total * sizeof(double);
+        for (int i = 0; i < 3; ++i) {
+            if (hipMalloc(&c->Unext[i], bytes) != hipSuccess) {
+                for (int k = 0; k <= i; ++k) {
+                    (void)hipFree(c->Unext[k]);
+                    c->Unext[k] = nullptr;
+                }
+                return fail(c, SWE_ERR_OOM, "fused steps: second state buffer");
+            }
+            SWE_CHECK(c, swe::launch_fill_double(c->Unext[i], 0.0, (long)c->n_total, c->stream));
+        }
+    }
+    c->fused_rows = std::max(4, std::min(rows, c->G.nx));
+    return SWE_OK;
+}
+
+int swe_hip_get_step_fusion(const swe_hip_ctx *c) { return c ? c->fused_rows : -1; }
 
 int swe_hip_get_scheme(const swe_hip_ctx *c, int *scheme, int *recon, int *riemann) {
     if (!c) return SWE_ERR_INVALID;

@@ -20,6 +20,11 @@ struct swe_hip_ctx {
     double *U[3] = {nullptr, nullptr, nullptr};
     double *U1[3] = {nullptr, nullptr, nullptr};
     double *U2[3] = {nullptr, nullptr, nullptr};
+    // fused steps (one launch per step): the state alternates between U and Unext, swapped
+    // after every launch so that U is always the current state; fused_rows = rows per tile,
+    // 0 = off
+    double *Unext[3] = {nullptr, nullptr, nullptr};
+    int fused_rows = 0;
     double *B = nullptr;
     double *scal = nullptr;      // [0] dt, [1] max speed
     double *partial = nullptr;   // per-block maxima

@@ -110,6 +110,18 @@ hipError_t launch_stage(int stage, int bc, bool speed, bool fast, const GridDev 
 // Grid of a strip x row-segment launch with `block_waves` waves (strips) per workgroup.
 dim3 stage_grid(const GridDev &G, int tile_rows, int block_waves = kStageBlockWaves);
 
+// One launch per SSP-RK3 step (the three stages fused, temporal blocking): A1 = stage 1
+// (in U^n, out U1), A2 = stage 2 (in U1, u0 U^n, out U2), A3 = stage 3 (in U2, u0 U^n, out
+// U^{n+1}, a fourth array set: never U^n); A1 carries dt, the status words, tile_rows (>= 4)
+// and nstrips = fused_strips_for(ny) (52-column strips); parity = the launch's index in the
+// call mod 2 (status words, see fused_step_kernel).  ng must be 2.  speed: per-block max wave
+// speed of U^{n+1} into A3.partial (fused_step_blocks entries).
+int fused_strips_for(int ny);
+int fused_step_blocks(const GridDev &G, int tile_rows);
+hipError_t launch_fused_step(int bc, bool speed, bool fast, const GridDev &G,
+                             const StageArgs &A1, const StageArgs &A2, const StageArgs &A3,
+                             int parity, hipStream_t stream, dim3 *grid_out);
+
 // Modular serial scheme family (swe_modular.hip): bathy = the hydrostatic flux assembly
 // (HLL only, as the reference dispatches it), else the plain one; recon 0 MUSCL / 1
 // piecewise constant; riemann 0 HLL / 1 Rusanov / 2 ROE.  Reflecting walls, fixed dt.

@@ -239,22 +239,25 @@ struct March {
 #define SWE_BATCH_WAVES_PER_EU SWE_STAGE_WAVES_PER_EU
 #endif
 
-// One wave's tile: strip `strip` (60 output columns) of rows [r0, min(r0 + tile_rows,
-// ng + nx)).  Accumulates the max wave speed of the written cells (SPEED) and a non-finite
-// flag into the caller's per-lane values.
+// One wave's march over output rows [r0, r1) (interior rows, r0 < r1) with lane l on
+// column col0 + l, writing the columns of lanes [own_lo, own_hi) that are interior.  Input
+// columns outside the 2-cell ghost frame are clamped to its edge: such lanes feed only
+// lanes outside [own_lo, own_hi) (the caller's halo), never a stored value.  Accumulates
+// the max wave speed of the written cells (SPEED) and a non-finite flag into the caller's
+// per-lane values.
 template <int STAGE, int BC, bool SPEED, bool FAST>
-__device__ __forceinline__ void run_tile(const GridDev &G, const StageArgs &A, long long base,
-                                         double dt, int strip, int r0, int lane, double &speed,
-                                         int &bad) {
+__device__ __forceinline__ void run_rows(const GridDev &G, const StageArgs &A, long long base,
+                                         double dt, int col0, int own_lo, int own_hi, int r0,
+                                         int r1, int lane, double &speed, int &bad) {
     const int ng = G.ng, nx = G.nx, ny = G.ny;
-    const int j = ng - 2 + strip * kStripCols + lane;
+    const int j = col0 + lane;
     bool gcol;
-    const int jc = mirror<BC>(j < ng + ny + 2 ? j : ng + ny + 1, ng, ng + ny, gcol);
-    const int r1 = (r0 + A.tile_rows < ng + nx) ? r0 + A.tile_rows : ng + nx;
+    const int jl = j < ng - 2 ? ng - 2 : (j < ng + ny + 2 ? j : ng + ny + 1);
+    const int jc = mirror<BC>(jl, ng, ng + ny, gcol);
 
     March<STAGE, BC, SPEED, FAST> m{A, base, G.stride, ng, nx, ng + nx + 1, r1,
-                                    (unsigned)j * 8u, (unsigned)jc * 8u, gcol,
-                                    lane >= 2 && lane < kWave - 2 && j < ng + ny,
+                                    (unsigned)jl * 8u, (unsigned)jc * 8u, gcol,
+                                    lane >= own_lo && lane < own_hi && j >= ng && j < ng + ny,
                                     G.half_g, G.g, G.inv_dx, G.inv_dy, dt, speed, bad,
                                     G.split_ok != 0};
 
@@ -296,6 +299,17 @@ __device__ __forceinline__ void run_tile(const GridDev &G, const StageArgs &A, l
     }
     speed = m.speed;
     bad = m.bad;
+}
+
+// One wave's tile of a stage launch: strip `strip` (60 output columns) of rows
+// [r0, min(r0 + tile_rows, ng + nx)).
+template <int STAGE, int BC, bool SPEED, bool FAST>
+__device__ __forceinline__ void run_tile(const GridDev &G, const StageArgs &A, long long base,
+                                         double dt, int strip, int r0, int lane, double &speed,
+                                         int &bad) {
+    const int r1 = (r0 + A.tile_rows < G.ng + G.nx) ? r0 + A.tile_rows : G.ng + G.nx;
+    run_rows<STAGE, BC, SPEED, FAST>(G, A, base, dt, G.ng - 2 + strip * kStripCols, 2,
+                                     kWave - 2, r0, r1, lane, speed, bad);
 }
 
 // Workgroup -> tile order.  The dispatcher hands consecutive workgroup ids of a 1-D grid
@@ -375,6 +389,88 @@ void stage_kernel(const GridDev G, const StageArgs A) {
                 for (int w = 1; w < kStageBlockWaves; ++w) mx = smax(mx, red[w]);
                 A.partial[blockIdx.x] = mx;
             }
+        }
+    }
+}
+
+// ---- one launch per step: the three stages fused (temporal blocking) -------------------
+//
+// A wave owns kFusedCols = 52 output columns and a segment of rows [r0, r1), and runs the
+// three stages of the step over it one after the other, each with the march above:
+//   stage 1 over rows [r0-4, r1+4) -> U1, writing lanes [2, 62)  (columns c0-4 .. c0+55)
+//   stage 2 over rows [r0-2, r1+2) -> U2, writing lanes [4, 60)  (columns c0-2 .. c0+53)
+//   stage 3 over rows [r0, r1)      -> U^{n+1}, writing lanes [6, 58) (columns c0 .. c0+51)
+// (rows clipped to the interior; lane l on column c0 - 6 + l).  Each stage widens the
+// stencil by 2 cells on every side, so every stage-2 and stage-3 value a wave reads is one
+// it wrote itself in the stage before: the ghost columns and rows it reads are mirrored
+// loads of interior cells it wrote, and the lanes beyond that only feed its own halo
+// lanes.  Neighbouring tiles overlap and write the same cells of U1 and U2 -- with the
+// same bits, as every cell's arithmetic is the same in every tile -- so no tile depends on
+// another; a workgroup-scope fence orders each wave's stores before its own loads of the
+// next stage (other lanes' cells, through the mirrored loads).  U^{n+1} goes to a separate
+// array (U^n is still read by other tiles' halos), and the caller swaps the two.
+//
+// Per output cell this computes 64/52 lanes x (3 + 12/(r1-r0)) row-stages of face
+// arithmetic against 64/60 x 3 for the three launches; the HBM traffic falls to U^n and B
+// read once, U^{n+1} written once, plus U1 and U2 written (and read back from cache).
+//
+// Run status (kStatusWords): launch k of a call writes word 1 + (k & 1) -- bit s-1 when
+// stage s produced a non-finite value, kFusedSkipped when it did nothing -- and reads the
+// other word, written by launch k-1: one set, and it does nothing but mark its own word.
+// A launch never reads the word it writes, so the failing launch runs to its end.
+constexpr int kFusedCols = kWave - 12;
+constexpr int kFusedSkipped = 8;
+
+template <int BC, bool SPEED, bool FAST>
+__global__ __launch_bounds__(kWave * kStageBlockWaves)
+__attribute__((amdgpu_waves_per_eu(SWE_STAGE_WAVES_PER_EU)))
+void fused_step_kernel(const GridDev G, const StageArgs A1, const StageArgs A2,
+                       const StageArgs A3, int parity) {
+    const double dt = *A1.dt;
+    int *const status = A1.flag;
+    const int stop = status[2 - parity];
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int groups = (A1.nstrips + kStageBlockWaves - 1) / kStageBlockWaves;
+    const int t = xcd_tile((int)blockIdx.x, (int)gridDim.x);
+    const int seg = t / groups;
+    const int strip = (t - seg * groups) * kStageBlockWaves + wave;
+    if (stop) {  // an earlier step of this call went non-finite
+        if (threadIdx.x == 0) atomicOr(status + 1 + parity, kFusedSkipped);
+        return;
+    }
+    if (blockIdx.x == 0 && threadIdx.x == 0) atomicAdd(status, 1);
+
+    double speed = 0.0;
+    int bad1 = 0, bad2 = 0, bad3 = 0;
+    if (strip < A1.nstrips) {
+        const int ng = G.ng, ie = G.ng + G.nx;
+        const int col0 = ng - 6 + strip * kFusedCols;
+        const int r0 = ng + seg * A1.tile_rows;
+        const int r1 = (r0 + A1.tile_rows < ie) ? r0 + A1.tile_rows : ie;
+        double unused = 0.0;
+        run_rows<1, BC, false, FAST>(G, A1, 0, dt, col0, 2, kWave - 2,
+                                     r0 - 4 > ng ? r0 - 4 : ng, r1 + 4 < ie ? r1 + 4 : ie,
+                                     lane, unused, bad1);
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+        run_rows<2, BC, false, FAST>(G, A2, 0, dt, col0, 4, kWave - 4,
+                                     r0 - 2 > ng ? r0 - 2 : ng, r1 + 2 < ie ? r1 + 2 : ie,
+                                     lane, unused, bad2);
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
+        run_rows<3, BC, SPEED, FAST>(G, A3, 0, dt, col0, 6, kWave - 6, r0, r1, lane, speed,
+                                     bad3);
+    }
+    const int bits = (__any(bad1) ? 1 : 0) | (__any(bad2) ? 2 : 0) | (__any(bad3) ? 4 : 0);
+    if (bits != 0 && lane == 0) atomicOr(status + 1 + parity, bits);
+    if (SPEED) {
+        for (int off = kWave / 2; off > 0; off >>= 1) speed = smax(speed, __shfl_xor(speed, off));
+        __shared__ double red[kStageBlockWaves];
+        if (lane == 0) red[wave] = speed;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            double mx = red[0];
+            for (int w = 1; w < kStageBlockWaves; ++w) mx = smax(mx, red[w]);
+            A3.partial[blockIdx.x] = mx;
         }
     }
 }
@@ -644,6 +740,36 @@ hipError_t launch_stage(int stage, int bc, bool speed, bool fast, const GridDev 
                        : launch_bc<1, true>(stage, speed, G, A, stream, grid);
     return bc == 0 ? launch_bc<0, false>(stage, speed, G, A, stream, grid)
                    : launch_bc<1, false>(stage, speed, G, A, stream, grid);
+}
+
+int fused_strips_for(int ny) { return (ny + kFusedCols - 1) / kFusedCols; }
+
+int fused_step_blocks(const GridDev &G, int tile_rows) {
+    const long long groups = (fused_strips_for(G.ny) + kStageBlockWaves - 1) / kStageBlockWaves;
+    return (int)(groups * ((G.nx + tile_rows - 1) / tile_rows));
+}
+
+hipError_t launch_fused_step(int bc, bool speed, bool fast, const GridDev &G,
+                             const StageArgs &A1, const StageArgs &A2, const StageArgs &A3,
+                             int parity, hipStream_t stream, dim3 *grid_out) {
+    if (G.ng != 2 || A1.tile_rows < 4 || A1.nstrips != fused_strips_for(G.ny) ||
+        (parity & ~1) != 0 || A3.out_h == A3.u0_h || A3.out_h == A3.in_h)
+        return hipErrorInvalidValue;
+    const dim3 grid((unsigned)fused_step_blocks(G, A1.tile_rows), 1, 1);
+    if (grid_out) *grid_out = grid;
+    const dim3 block(kWave * kStageBlockWaves);
+#define SWE_FUSED(BCV, SP, FA)                                                              \
+    hipLaunchKernelGGL((fused_step_kernel<BCV, SP, FA>), grid, block, 0, stream, G, A1, A2, A3, \
+                       parity)
+    if (bc == 0) {
+        if (fast) { if (speed) SWE_FUSED(0, true, true); else SWE_FUSED(0, false, true); }
+        else { if (speed) SWE_FUSED(0, true, false); else SWE_FUSED(0, false, false); }
+    } else {
+        if (fast) { if (speed) SWE_FUSED(1, true, true); else SWE_FUSED(1, false, true); }
+        else { if (speed) SWE_FUSED(1, true, false); else SWE_FUSED(1, false, false); }
+    }
+#undef SWE_FUSED
+    return hipGetLastError();
 }
 
 hipError_t launch_batch_stage(int stage, int bc, bool speed, bool fast, const BatchArgs &B,

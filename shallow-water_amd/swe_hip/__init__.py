@@ -68,6 +68,8 @@ _SIGS = {
     "swe_hip_error_string": (C.c_char_p, [C.c_int]),
     "swe_hip_last_error": (C.c_char_p, [C.c_void_p]),
     "swe_hip_last_status": (C.c_int, [C.c_void_p, C.POINTER(C.c_int64), C.POINTER(C.c_int)]),
+    "swe_hip_set_step_fusion": (C.c_int, [C.c_void_p, C.c_int]),
+    "swe_hip_get_step_fusion": (C.c_int, [C.c_void_p]),
     "swe_hip_device_count": (C.c_int, []),
     "swe_hip_create": (C.c_int, [C.POINTER(GridDesc), C.c_int, C.POINTER(C.c_void_p)]),
     "swe_hip_destroy": (None, [C.c_void_p]),
@@ -252,6 +254,15 @@ class HipSolver:
         self.last_dt_hist = hist[:nsteps]
         self._check(lib().swe_hip_step_adaptive(self._ctx, float(cfl), int(nsteps), _p(hist)))
         return hist[:nsteps]
+
+    @property
+    def step_fusion(self):
+        """Rows per tile of the fused one-launch step (0: three launches per step)."""
+        return lib().swe_hip_get_step_fusion(self._ctx)
+
+    @step_fusion.setter
+    def step_fusion(self, rows):
+        self._check(lib().swe_hip_set_step_fusion(self._ctx, int(rows)))
 
     def last_status(self):
         """(steps started, failing stage or 0) of the last stepping call: after
