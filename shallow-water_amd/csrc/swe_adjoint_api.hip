@@ -510,6 +510,9 @@ int backward_common(swe_hip_adj *a, const double *lh, const double *lhu, const d
         ADJ_CHECK(a, swe::launch_zero_ghosts(c->G, a->lam[0][k], c->stream));
     }
     ADJ_CHECK(a, swe::launch_fill_double(a->lam_b, 0.0, (long)a->n_total, c->stream));
+    // the recomputing stage launches honour the first-bad-stage stop: clear what a
+    // stepping call on the context may have left since the forward
+    ADJ_CHECK(a, swe::launch_fill_int(c->flag, 0, swe::kStatusWords, c->stream));
     a->parity = 0;
     ADJ_CHECK(a, swe::launch_floor_mask(c->G, c->U[0], c->U[1], c->U[2], a->mask[0], c->stream));
     a->state = 2;
