@@ -88,6 +88,24 @@ def test_fused_adaptive_history_and_state(hip):
         assert same(g_, w_)
 
 
+@pytest.mark.parametrize("bc", [0, 1])
+def test_fused_fast_precision_and_adaptive_transmissive(hip, bc):
+    """The FAST precision mode (not bitwise to the reference, but the same arithmetic in
+    both launch modes) and adaptive dt with either wall type: the fused step equals the
+    three launches bitwise."""
+    c = W.random_state(77, 131, seed=8, bc=bc)
+    runs = []
+    for rows in (0, 12):
+        with solver(hip, c, rows) as s:
+            s.precision = hip.PRECISION_FAST
+            hist = s.step_adaptive(0.35, 7)
+            runs.append((hist, s.download()))
+    (h0, a0), (h1, a1) = runs
+    assert same(h0, h1)
+    for g_, w_ in zip(a1, a0):
+        assert same(g_, w_)
+
+
 def test_fused_across_calls_and_modes(hip):
     """Odd and even step counts (the state buffers swap every step), then back to three
     launches (a fresh graph on the current buffers), then fused again."""
