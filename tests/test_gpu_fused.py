@@ -127,6 +127,32 @@ def test_fused_across_calls_and_modes(hip):
         assert same(g_, w_)
 
 
+def test_fused_mixed_with_other_entry_points(hip):
+    """Zero-step calls, the allocation-free adaptive step and the right-hand side between
+    fused calls all act on the current state buffer."""
+    torch = pytest.importorskip("torch")
+    c = W.gauss_pulse(80, g=hip.G_REF)
+    with solver(hip, c) as s:
+        s.step(2e-4, 3)
+        h1 = s.step_adaptive(0.4, 4)
+        r1 = s.rhs()
+        s.step(2e-4, 2)
+        want = s.download()
+    with solver(hip, c, 16) as s:
+        s.step(2e-4, 0)
+        s.step(2e-4, 3)
+        hist = torch.zeros(4, dtype=torch.float64, device="cuda")
+        s.step_adaptive_noalloc(0.4, 4, hist)
+        r2 = s.rhs()
+        s.step(2e-4, 2)
+        got = s.download()
+    assert same(hist.cpu().numpy(), h1)
+    for a, b in zip(r2, r1):
+        assert same(a, b)
+    for g_, w_ in zip(got, want):
+        assert same(g_, w_)
+
+
 @pytest.mark.parametrize("name", sorted(STOP_CASES))
 def test_fused_first_bad_stage_stop(hip, name):
     """The fused launch that finds a non-finite stage output runs to its end, every later
