@@ -103,7 +103,13 @@ struct March {
                   ld(row_ptr(A.u0_hv, ro), jc_bytes)};
     }
 
-    __device__ __forceinline__ void row(int r, const Cell &a, const Cell &b, Cell &n,
+    // SPLIT_ONLY: the faces take the split forms without a test, and the step returns
+    // false, before the x face and the stores, when the newly loaded row r+2 fails the
+    // range guard (the x face of row r reads row r+2 through the slope of row r+1).
+    // Otherwise each face takes the split forms while the guard holds, and the step always
+    // completes (returns true).
+    template <bool SPLIT_ONLY>
+    __device__ __forceinline__ bool row(int r, const Cell &a, const Cell &b, Cell &n,
                                         const V3 &sa, V3 &sb, const V3 &d2, V3 &d3,
                                         const V3 &fxp, V3 &fxn) {
         // Row r+2 and U^n of row r are loaded first and not read before the y faces of row
@@ -115,17 +121,19 @@ struct March {
         if (STAGE >= 2) u0 = load_u0(r);
         // FAST has its own reciprocal arithmetic; only the exact mode has a split variant.
         FaceFlux fy, fx;
-        if (!FAST && split)
+        if (SPLIT_ONLY || (!FAST && split))
             fy = y_faces<true>(a);
         else
             fy = y_faces<false>(a);
         finish(n, gr);
+        if (SPLIT_ONLY && !split) return false;
         d3 = V3{n.eta - b.eta, n.hu - b.hu, n.hv - b.hv};
-        if (!FAST && split)
+        if (SPLIT_ONLY || (!FAST && split))
             fx = x_face<true>(a, b, sa, sb, d2, d3);
         else
             fx = x_face<false>(a, b, sa, sb, d2, d3);
         tail(r, a, fx, fy, fxp, fxn, u0);
+        return true;
     }
 
     // Half the limited slope, 0.5 * minmod(a, b) (omp_kernels.hpp:146-152, :214-220): the
@@ -239,12 +247,83 @@ struct March {
 #define SWE_BATCH_WAVES_PER_EU SWE_STAGE_WAVES_PER_EU
 #endif
 
+// The march of one wave over output rows [r0, r1) from a fresh prologue (rows r0-2 .. r0+1
+// and the plus-side flux of face r0-1/2).  Returns r1 when it finished, or the first row
+// it did not store.
+//  SPLIT_ONLY: every face takes the split division/sqrt forms (exact mode, range guard
+//    held so far); the march stops at the prologue or before the x face of the first row
+//    whose newly loaded row fails the guard.  The loop holds only the split face bodies,
+//    so its code stays small for the instruction cache.
+//  otherwise: the faces take the split forms while the guard holds and the full IEEE
+//    sequences after it fails (same values either way), and the march always finishes.
+// UNROLL: the loop is unrolled 6x so that the 3-row window and the 2-deep slope,
+// difference and flux pairs rotate by renaming; without it they rotate by copies (the
+// exact mode's fallback march, cold code).
+template <bool SPLIT_ONLY, bool UNROLL, int STAGE, int BC, bool SPEED, bool FAST>
+__device__ __forceinline__ int march(March<STAGE, BC, SPEED, FAST> &m, int r0, int r1) {
+    const Cell P2 = m.load(r0 - 2);
+    const Cell P1 = m.load(r0 - 1);
+    Cell C0 = m.load(r0), C1 = m.load(r0 + 1), C2;
+    if (SPLIT_ONLY && !m.split) return r0;
+
+    // x differences d_{r0-3/2}, d_{r0-1/2}, d_{r0+1/2}
+    const V3 dm{P1.eta - P2.eta, P1.hu - P2.hu, P1.hv - P2.hv};
+    const V3 d0{C0.eta - P1.eta, C0.hu - P1.hu, C0.hv - P1.hv};
+    V3 D0{C1.eta - C0.eta, C1.hu - C0.hu, C1.hv - C0.hv}, D1;
+    // minus-side reconstruction of row r0-1 at face r0-1/2, slope of row r0
+    const V3 sp{0.5 * minmod(dm.e, d0.e), 0.5 * minmod(dm.u, d0.u), 0.5 * minmod(dm.v, d0.v)};
+    V3 S0{0.5 * minmod(d0.e, D0.e), 0.5 * minmod(d0.u, D0.u), 0.5 * minmod(d0.v, D0.v)}, S1;
+    const FaceFlux f0 =
+        (SPLIT_ONLY || (!FAST && m.split))
+            ? face_flux<FAST, true>(m.half_g, m.g, P1.eta + sp.e, P1.hu + sp.u, P1.hv + sp.v,
+                                    P1.b, C0.eta - S0.e, C0.hu - S0.u, C0.hv - S0.v, C0.b)
+            : face_flux<FAST, false>(m.half_g, m.g, P1.eta + sp.e, P1.hu + sp.u, P1.hv + sp.v,
+                                     P1.b, C0.eta - S0.e, C0.hu - S0.u, C0.hv - S0.v, C0.b);
+    V3 F0{f0.fh, f0.fn_p, f0.ft}, F1;
+
+    if constexpr (UNROLL) {
+        for (int r = r0; r < r1; r += 6) {
+            if (!m.template row<SPLIT_ONLY>(r, C0, C1, C2, S0, S1, D0, D1, F0, F1)) return r;
+            if (r + 1 >= r1) break;
+            if (!m.template row<SPLIT_ONLY>(r + 1, C1, C2, C0, S1, S0, D1, D0, F1, F0))
+                return r + 1;
+            if (r + 2 >= r1) break;
+            if (!m.template row<SPLIT_ONLY>(r + 2, C2, C0, C1, S0, S1, D0, D1, F0, F1))
+                return r + 2;
+            if (r + 3 >= r1) break;
+            if (!m.template row<SPLIT_ONLY>(r + 3, C0, C1, C2, S1, S0, D1, D0, F1, F0))
+                return r + 3;
+            if (r + 4 >= r1) break;
+            if (!m.template row<SPLIT_ONLY>(r + 4, C1, C2, C0, S0, S1, D0, D1, F0, F1))
+                return r + 4;
+            if (r + 5 >= r1) break;
+            if (!m.template row<SPLIT_ONLY>(r + 5, C2, C0, C1, S1, S0, D1, D0, F1, F0))
+                return r + 5;
+        }
+    } else {
+#pragma clang loop unroll(disable)
+        for (int r = r0; r < r1; ++r) {
+            if (!m.template row<SPLIT_ONLY>(r, C0, C1, C2, S0, S1, D0, D1, F0, F1)) return r;
+            C0 = C1;
+            C1 = C2;
+            S0 = S1;
+            D0 = D1;
+            F0 = F1;
+        }
+    }
+    return r1;
+}
+
 // One wave's march over output rows [r0, r1) (interior rows, r0 < r1) with lane l on
 // column col0 + l, writing the columns of lanes [own_lo, own_hi) that are interior.  Input
 // columns outside the 2-cell ghost frame are clamped to its edge: such lanes feed only
 // lanes outside [own_lo, own_hi) (the caller's halo), never a stored value.  Accumulates
 // the max wave speed of the written cells (SPEED) and a non-finite flag into the caller's
 // per-lane values.
+//
+// In the exact mode the split-only march runs first; if the range guard fails in a row,
+// the fallback march restarts from that row with its own prologue (the rows before it are
+// stored, and the faces it recomputes have the same values in either form).
 template <int STAGE, int BC, bool SPEED, bool FAST>
 __device__ __forceinline__ void run_rows(const GridDev &G, const StageArgs &A, long long base,
                                          double dt, int col0, int own_lo, int own_hi, int r0,
@@ -260,42 +339,11 @@ __device__ __forceinline__ void run_rows(const GridDev &G, const StageArgs &A, l
                                     lane >= own_lo && lane < own_hi && j >= ng && j < ng + ny,
                                     G.half_g, G.g, G.inv_dx, G.inv_dy, dt, speed, bad,
                                     G.split_ok != 0};
-
-    // ---- prologue: rows r0-2 .. r0+1 and the plus-side flux of face r0-1/2
-    const Cell P2 = m.load(r0 - 2);
-    const Cell P1 = m.load(r0 - 1);
-    Cell C0 = m.load(r0), C1 = m.load(r0 + 1), C2;
-
-    // x differences d_{r0-3/2}, d_{r0-1/2}, d_{r0+1/2}
-    const V3 dm{P1.eta - P2.eta, P1.hu - P2.hu, P1.hv - P2.hv};
-    const V3 d0{C0.eta - P1.eta, C0.hu - P1.hu, C0.hv - P1.hv};
-    V3 D0{C1.eta - C0.eta, C1.hu - C0.hu, C1.hv - C0.hv}, D1;
-    // minus-side reconstruction of row r0-1 at face r0-1/2, slope of row r0
-    const V3 sp{0.5 * minmod(dm.e, d0.e), 0.5 * minmod(dm.u, d0.u), 0.5 * minmod(dm.v, d0.v)};
-    V3 S0{0.5 * minmod(d0.e, D0.e), 0.5 * minmod(d0.u, D0.u), 0.5 * minmod(d0.v, D0.v)}, S1;
-    const FaceFlux f0 =
-        (!FAST && m.split) ? face_flux<FAST, true>(m.half_g, m.g, P1.eta + sp.e, P1.hu + sp.u,
-                                                   P1.hv + sp.v, P1.b, C0.eta - S0.e,
-                                                   C0.hu - S0.u, C0.hv - S0.v, C0.b)
-                           : face_flux<FAST, false>(m.half_g, m.g, P1.eta + sp.e, P1.hu + sp.u,
-                                                    P1.hv + sp.v, P1.b, C0.eta - S0.e,
-                                                    C0.hu - S0.u, C0.hv - S0.v, C0.b);
-    V3 F0{f0.fh, f0.fn_p, f0.ft}, F1;
-
-    // ---- march, unrolled 6x (the 3-row window and the 2-deep slope, difference and flux
-    // pairs) so every register window rotates by renaming
-    for (int r = r0; r < r1; r += 6) {
-        m.row(r, C0, C1, C2, S0, S1, D0, D1, F0, F1);
-        if (r + 1 >= r1) break;
-        m.row(r + 1, C1, C2, C0, S1, S0, D1, D0, F1, F0);
-        if (r + 2 >= r1) break;
-        m.row(r + 2, C2, C0, C1, S0, S1, D0, D1, F0, F1);
-        if (r + 3 >= r1) break;
-        m.row(r + 3, C0, C1, C2, S1, S0, D1, D0, F1, F0);
-        if (r + 4 >= r1) break;
-        m.row(r + 4, C1, C2, C0, S0, S1, D0, D1, F0, F1);
-        if (r + 5 >= r1) break;
-        m.row(r + 5, C2, C0, C1, S1, S0, D1, D0, F1, F0);
+    int r = r0;
+    if (!FAST && G.split_ok) r = march<true, true>(m, r0, r1);
+    if (r < r1) {
+        m.split = G.split_ok != 0;
+        march<false, true>(m, r, r1);
     }
     speed = m.speed;
     bad = m.bad;
