@@ -465,7 +465,7 @@ def main():
     s.upload(case["h"], case["hu"], case["hv"], case["B"])
     dt, max_speed = s.stable_dt(0.45 if args.workload == "dam_break" else 0.4)
     m0 = mass(case["h"], case["ng"])
-    log(f"[bench] dt={dt!r} max_speed={max_speed!r} tile_rows={s.tile_rows}")
+    log(f"[bench] dt={dt!r} max_speed={max_speed!r} stage tile rows={s.stage_tile_rows}")
 
     # warmup (also builds the step graph)
     s.step(dt, max(args.warmup, 1))
@@ -524,7 +524,7 @@ def main():
                                 if args.workload == "dam_break" else
                                 f"wet_bathymetry_{n}x{n}_fp64_reflecting_cfl0.4"),
                    "nx": n, "ny": n, "ng": 2, "dt": dt, "parallelism": "single GPU",
-                   "tile_rows": s.tile_rows, "precision": args.precision},
+                   "tile_rows": list(s.stage_tile_rows), "precision": args.precision},
         "roofline": {
             "bound": "hbm", "kernel": "stage_kernel<2> (SSP-RK3 stage 2, 80 B/cell)",
             "achieved": round(ach_dom, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",

@@ -165,6 +165,12 @@ int swe_hip_rhs(swe_hip_ctx *ctx, double *rhs_h, double *rhs_hu, double *rhs_hv)
  * grid size. */
 int swe_hip_set_tile_rows(swe_hip_ctx *ctx, int tile_rows);
 int swe_hip_get_tile_rows(const swe_hip_ctx *ctx);
+/* Tile height of one stage's launches (stage 1..3; swe_hip_set_tile_rows sets all three,
+ * swe_hip_get_tile_rows returns stage 1's; the right-hand side alone runs with stage 1's).
+ * The default gives stages 2 and 3 shorter tiles than stage 1 on large grids.  Results do
+ * not depend on it.  No counterpart in the reference (a launch-shape setting). */
+int swe_hip_set_stage_tile_rows(swe_hip_ctx *ctx, int stage, int tile_rows);
+int swe_hip_get_stage_tile_rows(const swe_hip_ctx *ctx, int stage);
 
 /* Arithmetic of the stage kernels (swe_precision); default SWE_PRECISION_EXACT.  No
  * counterpart in the reference: FAST trades bitwise parity for fewer FP64 divisions. */

@@ -229,7 +229,7 @@ swe::StageArgs fwd_args(swe_hip_adj *a, const State &in, const State &u0, const 
     s.dt = dt;
     s.partial = nullptr;
     s.flag = c->flag;
-    s.tile_rows = c->tile_rows;
+    s.tile_rows = c->stage_rows[0];
     s.nstrips = swe::strips_for(c->G.ny);
     return s;
 }

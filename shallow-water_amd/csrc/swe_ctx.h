@@ -35,16 +35,18 @@ struct swe_hip_ctx {
     long long *counter = nullptr;
     double *hist = nullptr;
     long long hist_cap = 0;
-    int tile_rows = 64;
+    // interior rows per wave of the stage launches, per stage (stage 0, the right-hand side
+    // alone, takes stage 1's)
+    int stage_rows[3] = {64, 64, 64};
     int precision = SWE_PRECISION_EXACT;
     int scheme = SWE_SCHEME_OPTIMIZED;  // swe_scheme; recon / riemann for the serial family
     int recon = SWE_RECON_MUSCL;
     int riemann = SWE_RIEMANN_HLL;
     bool use_graph = true;
-    // cached graph of one step: key = (mode, tile_rows, stream, hist pointer)
+    // cached graph of one step: key = (mode, stage tile heights, stream, hist pointer)
     hipGraphExec_t graph = nullptr;
     int graph_mode = -1;
-    int graph_tile = -1;
+    int graph_rows[3] = {-1, -1, -1};
     int graph_precision = -1;
     int graph_scheme = -1;  // scheme * 16 + recon * 4 + riemann of the captured step
     hipStream_t graph_stream = nullptr;

@@ -86,6 +86,8 @@ _SIGS = {
     "swe_hip_rhs": (C.c_int, [C.c_void_p, _dp, _dp, _dp]),
     "swe_hip_set_tile_rows": (C.c_int, [C.c_void_p, C.c_int]),
     "swe_hip_get_tile_rows": (C.c_int, [C.c_void_p]),
+    "swe_hip_set_stage_tile_rows": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
+    "swe_hip_get_stage_tile_rows": (C.c_int, [C.c_void_p, C.c_int]),
     "swe_hip_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
     "swe_hip_get_precision": (C.c_int, [C.c_void_p]),
     "swe_hip_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -294,6 +296,16 @@ class HipSolver:
     @tile_rows.setter
     def tile_rows(self, v):
         self._check(lib().swe_hip_set_tile_rows(self._ctx, int(v)))
+
+    @property
+    def stage_tile_rows(self):
+        """Tile heights of the three stage launches (stage 1, 2, 3)."""
+        return tuple(lib().swe_hip_get_stage_tile_rows(self._ctx, s) for s in (1, 2, 3))
+
+    @stage_tile_rows.setter
+    def stage_tile_rows(self, rows):
+        for s, v in zip((1, 2, 3), rows):
+            self._check(lib().swe_hip_set_stage_tile_rows(self._ctx, s, int(v)))
 
     @property
     def precision(self):

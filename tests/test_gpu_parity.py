@@ -217,6 +217,33 @@ def test_tile_rows_invariant(hip, tile_rows):
         assert_bitwise(g_, w_, f"tile_rows={tile_rows} {comp}")
 
 
+@pytest.mark.parametrize("rows", [(32, 24, 24), (5, 64, 3), (1000, 1, 17)])
+def test_stage_tile_rows_invariant(hip, rows):
+    """Each stage launch with its own tile height: fixed and adaptive steps bitwise equal
+    to the oracle, and the graph recaptured when the heights change."""
+    c = W.lake_wet_dry(97, seed=6, eta0=2.0)
+    gr, h, hu, hv, B = oracle_state(c)
+    dt, _ = oracle.stable_dt(gr, h, hu, hv, 0.4)
+    with gpu_solver(hip, c) as s:
+        s.step(dt, 2)
+        s.stage_tile_rows = rows
+        assert s.stage_tile_rows == tuple(min(r, c["nx"]) for r in rows)
+        assert s.tile_rows == min(rows[0], c["nx"])
+        s.step(dt, 5)
+        got = s.download()
+    oracle.run(gr, h, hu, hv, B, 7, dt=dt)
+    for comp, g_, w_ in zip(("h", "hu", "hv"), got, (h, hu, hv)):
+        assert_bitwise(g_, w_, f"stage rows {rows} {comp}")
+    hists = []
+    for r in (rows, (8, 8, 8)):
+        with gpu_solver(hip, c) as s:
+            s.stage_tile_rows = r
+            hists.append((s.step_adaptive(0.4, 6), s.download()))
+    assert np.array_equal(hists[0][0], hists[1][0])
+    for a, b in zip(hists[0][1], hists[1][1]):
+        assert_bitwise(a, b, f"adaptive, stage rows {rows} vs 8")
+
+
 def test_graph_and_direct_launch_agree(hip):
     c = W.ensemble_member(7, 90)
     outs = []
