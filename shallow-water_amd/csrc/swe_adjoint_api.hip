@@ -212,8 +212,9 @@ State state_of(swe_hip_adj *a, int slot) {
     return State{{slot_ptr(a, slot, 0), slot_ptr(a, slot, 1), slot_ptr(a, slot, 2)}};
 }
 
+// rows: tile height of the launch (0: the context's stage-1 height).
 swe::StageArgs fwd_args(swe_hip_adj *a, const State &in, const State &u0, const State &out,
-                        const double *dt) {
+                        const double *dt, int rows = 0) {
     swe_hip_ctx *c = a->ctx;
     swe::StageArgs s{};
     s.in_h = in.p[0];
@@ -229,10 +230,17 @@ swe::StageArgs fwd_args(swe_hip_adj *a, const State &in, const State &u0, const 
     s.dt = dt;
     s.partial = nullptr;
     s.flag = c->flag;
-    s.tile_rows = c->stage_rows[0];
+    s.tile_rows = rows > 0 ? std::min(rows, c->G.nx) : c->stage_rows[0];
     s.nstrips = swe::strips_for(c->G.ny);
     return s;
 }
+
+// Tile height of the segment recompute that runs beside the reversal on the second stream.
+// The default heights fill the GPU when a grid steps on its own (4 rows at 1024^2), at the
+// price of a 4-row prologue per tile; beside the adjoint kernels, which hold most wave
+// slots, fewer and taller tiles cost less: at 1024^2 the records plan's backward took
+// 560 / 574 / 575 / 596 ms with 8 / 4 / 16 / 32-row tiles (interleaved runs, one box).
+int recompute_rows(const swe_hip_ctx *c) { return std::max(8, c->stage_rows[0]); }
 
 // The context's stage buffers (stage states of a step that is not recorded).
 State scratch_stage(swe_hip_adj *a, int which) {
@@ -246,15 +254,16 @@ State rec_state(swe_hip_adj *a, int64_t r, int which) {
     return State{{base, base + a->n_total, base + 2 * a->n_total}};
 }
 
-// Stages 1 and 2 of step p from `base` into u1, u2 (on `st`; default the context's stream).
+// Stages 1 and 2 of step p from `base` into u1, u2 (on `st`; default the context's stream;
+// rows: tile height, 0 for the context's).
 int stages_12(swe_hip_adj *a, const State &base, const State &u1, const State &u2, int64_t p,
-              hipStream_t st = nullptr) {
+              hipStream_t st = nullptr, int rows = 0) {
     swe_hip_ctx *c = a->ctx;
     if (!st) st = c->stream;
     ADJ_CHECK(a, swe::launch_stage(1, c->bc, false, false, c->G,
-                                   fwd_args(a, base, base, u1, a->dts + p), st, nullptr));
+                                   fwd_args(a, base, base, u1, a->dts + p, rows), st, nullptr));
     ADJ_CHECK(a, swe::launch_stage(2, c->bc, false, false, c->G,
-                                   fwd_args(a, u1, base, u2, a->dts + p), st, nullptr));
+                                   fwd_args(a, u1, base, u2, a->dts + p, rows), st, nullptr));
     return SWE_OK;
 }
 
@@ -416,16 +425,17 @@ int backward_records(swe_hip_adj *a) {
         int64_t s0, s1;
         seg_range(seg, s0, s1);
         const State ck = state_of(a, (int)seg);
+        const int rows = st != c->stream ? recompute_rows(c) : 0;
         for (int64_t p = s0; p < s1; ++p) {
             const State base = p == s0 ? ck : rec_state(a, o + p - s0, 0);
             const State u1 = rec_state(a, o + p - s0, 1), u2 = rec_state(a, o + p - s0, 2);
-            const int rc = stages_12(a, base, u1, u2, p, st);
+            const int rc = stages_12(a, base, u1, u2, p, st, rows);
             if (rc) return rc;
             if (p + 1 < s1)  // the segment's last step needs only its stage states
                 ADJ_CHECK(a, swe::launch_stage(3, c->bc, false, false, c->G,
                                                fwd_args(a, u2, base,
                                                         rec_state(a, o + p + 1 - s0, 0),
-                                                        a->dts + p),
+                                                        a->dts + p, rows),
                                                st, nullptr));
         }
         return SWE_OK;
