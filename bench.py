@@ -40,7 +40,10 @@ import numpy as np  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0            # MI355X HBM3E peak, GB/s
 SHADER_CLOCK_GHZ = 2.4           # MI355X peak engine clock
-BYTES_PER_CELL_STAGE = (56, 80, 80)  # algorithmic HBM bytes per interior cell, stages 1..3
+# algorithmic HBM bytes per interior cell, stages 1..3: read U_s and B (and U^n for stages 2
+# and 3), write U_{s+1}; with a bottom of +0.0 everywhere the stage kernels do not read B
+BYTES_PER_CELL_STAGE = (56, 80, 80)
+BYTES_PER_CELL_STAGE_FLAT_B = (48, 72, 72)
 
 
 def log(*a):
@@ -481,11 +484,33 @@ def main():
     # per-stage event timing (direct launches, events on the solver stream)
     stage_ms, ev_total = s.time_steps(dt, args.event_steps)
     log(f"[bench] stage ms {stage_ms}, event total {ev_total:.3f} ms / {args.event_steps} steps")
+    flat_b = s.flat_bottom
+    bpc = BYTES_PER_CELL_STAGE_FLAT_B if flat_b else BYTES_PER_CELL_STAGE
+    general = None
+    steps_done = max(args.warmup, 1) + args.steps + args.event_steps
+    if flat_b:
+        # the same grid through the kernels that read B (the general path), for comparison
+        s.set_flat_bottom(False)
+        gsteps = max(1, min(args.steps, 300))
+        s.step(dt, 2)
+        s.sync()
+        t0 = time.perf_counter()
+        s.step(dt, gsteps)
+        gel = time.perf_counter() - t0
+        gev = max(1, min(args.event_steps, 50))
+        g_ms, _ = s.time_steps(dt, gev)
+        s.set_flat_bottom(True)
+        steps_done += 2 + gsteps + gev
+        general = {"gcell_per_s": round(cells * gsteps / gel / 1e9, 4), "steps": gsteps,
+                   "ms_per_step": round(gel / gsteps * 1e3, 4),
+                   "stage_ms": [round(x, 4) for x in g_ms],
+                   "bytes_per_cell_update": sum(BYTES_PER_CELL_STAGE)}
+        log(f"[bench] general (B read) path: {general['gcell_per_s']:.2f} Gcell/s, "
+            f"stage ms {g_ms}")
     h_end = s.download()[0]
     m1 = mass(h_end, case["ng"])
-    steps_done = max(args.warmup, 1) + args.steps + args.event_steps
-    stage_bytes = [b * cells for b in BYTES_PER_CELL_STAGE]
-    dom = 1  # stage 2 kernel: 80 B/cell, the largest per launch
+    stage_bytes = [b * cells for b in bpc]
+    dom = 1  # stage 2 kernel: 80 (72) B/cell, the largest per launch
     ach_dom = stage_bytes[dom] / (stage_ms[dom] * 1e-3) / 1e9
     step_bytes = sum(stage_bytes)
     ach_step = step_bytes / (sum(stage_ms) * 1e-3) / 1e9
@@ -524,9 +549,12 @@ def main():
                                 if args.workload == "dam_break" else
                                 f"wet_bathymetry_{n}x{n}_fp64_reflecting_cfl0.4"),
                    "nx": n, "ny": n, "ng": 2, "dt": dt, "parallelism": "single GPU",
-                   "tile_rows": list(s.stage_tile_rows), "precision": args.precision},
+                   "tile_rows": list(s.stage_tile_rows), "precision": args.precision,
+                   "bottom_not_read": bool(flat_b)},
         "roofline": {
-            "bound": "hbm", "kernel": "stage_kernel<2> (SSP-RK3 stage 2, 80 B/cell)",
+            "bound": "hbm",
+            "kernel": (f"stage_kernel<2> (SSP-RK3 stage 2, {bpc[1]} B/cell"
+                       + (": bottom +0.0 everywhere, not read)" if flat_b else ")")),
             "achieved": round(ach_dom, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(ach_dom / HBM_PEAK_GBS, 4), "traffic": traffic,
             # the roof that binds: FP64 VALU issue (the HBM fraction above is the metric's
@@ -535,10 +563,11 @@ def main():
             "valu": valu,
         },
         "roofline_step": {
-            "bytes_per_cell_update": sum(BYTES_PER_CELL_STAGE), "achieved": round(ach_step, 1),
+            "bytes_per_cell_update": sum(bpc), "achieved": round(ach_step, 1),
             "frac": round(ach_step / HBM_PEAK_GBS, 4), "stage_ms": [round(x, 4) for x in stage_ms],
         },
-        "pct_hbm_peak": round(100.0 * value * 1e9 * 216 / (HBM_PEAK_GBS * 1e9), 2),
+        "pct_hbm_peak": round(100.0 * value * 1e9 * sum(bpc) / (HBM_PEAK_GBS * 1e9), 2),
+        "general_bottom_path": general,
         "mass_rel_change": abs(m1 - m0) / m0,
         "steps_total_on_state": steps_done,
     }

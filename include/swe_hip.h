@@ -172,6 +172,14 @@ int swe_hip_get_tile_rows(const swe_hip_ctx *ctx);
 int swe_hip_set_stage_tile_rows(swe_hip_ctx *ctx, int stage, int tile_rows);
 int swe_hip_get_stage_tile_rows(const swe_hip_ctx *ctx, int stage);
 
+/* Bottom that is +0.0 in every cell (ghosts included, checked at every upload): with mode 1
+ * (the default) the exact-mode stage launches then take kernels that do not read B, with
+ * the same results; mode 0 always reads it.  get returns 1 when those kernels are in use.
+ * No counterpart in the reference (whose flat bathymetry is an array of zeros read like
+ * any other). */
+int swe_hip_set_flat_bottom(swe_hip_ctx *ctx, int mode);
+int swe_hip_get_flat_bottom(const swe_hip_ctx *ctx);
+
 /* Arithmetic of the stage kernels (swe_precision); default SWE_PRECISION_EXACT.  No
  * counterpart in the reference: FAST trades bitwise parity for fewer FP64 divisions. */
 int swe_hip_set_precision(swe_hip_ctx *ctx, int mode);

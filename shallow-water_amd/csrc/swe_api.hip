@@ -81,6 +81,9 @@ int ensure_partial(swe_hip_ctx *c, int need) {
     return SWE_OK;
 }
 
+// The stage launches take the kernels that do not read B (same values).
+bool flat_bottom(const swe_hip_ctx *c) { return c->flat_bottom_mode != 0 && c->b_zero; }
+
 StageArgs stage_args(swe_hip_ctx *c, int stage) {
     StageArgs a{};
     double *const *in = stage == 1 || stage == 0 ? c->U : (stage == 2 ? c->U1 : c->U2);
@@ -100,6 +103,7 @@ StageArgs stage_args(swe_hip_ctx *c, int stage) {
     a.partial = c->partial;
     a.flag = c->flag;
     a.tile_rows = c->stage_rows[stage <= 1 ? 0 : stage - 1];
+    a.flat_b = flat_bottom(c) ? 1 : 0;
     a.nstrips = swe::strips_for(c->G.ny);
     return a;
 }
@@ -174,6 +178,7 @@ int run_steps(swe_hip_ctx *c, int mode, double cfl, int64_t nsteps) {
     }
     if (!c->graph || c->graph_mode != mode || c->graph_rows[0] != c->stage_rows[0] ||
         c->graph_rows[1] != c->stage_rows[1] || c->graph_rows[2] != c->stage_rows[2] ||
+        c->graph_flat != (int)flat_bottom(c) ||
         c->graph_precision != c->precision || c->graph_stream != c->stream ||
         c->graph_scheme != scheme_key(c) ||
         (mode == 1 && c->graph_cfl != cfl)) {
@@ -194,6 +199,7 @@ int run_steps(swe_hip_ctx *c, int mode, double cfl, int64_t nsteps) {
         SWE_CHECK(c, ei);
         c->graph_mode = mode;
         for (int i = 0; i < 3; ++i) c->graph_rows[i] = c->stage_rows[i];
+        c->graph_flat = (int)flat_bottom(c);
         c->graph_precision = c->precision;
         c->graph_scheme = scheme_key(c);
         c->graph_stream = c->stream;
@@ -281,7 +287,13 @@ int upload_common(swe_hip_ctx *c, const double *h, const double *hu, const doubl
     // B ghosts as at solver construction (omp_solver.hpp:119); state ghosts rebuilt too.
     SWE_CHECK(c, swe::launch_ghost_fill(c->bc, c->G, c->B, nullptr, nullptr, c->stream));
     SWE_CHECK(c, swe::launch_ghost_fill(c->bc, c->G, c->U[0], c->U[1], c->U[2], c->stream));
+    // is the bottom +0.0 everywhere (ghosts included)?
+    SWE_CHECK(c, swe::launch_fill_int(c->bz, 0, 1, c->stream));
+    SWE_CHECK(c, swe::launch_nonzero_bits(c->B, (long)c->n_total, c->bz, c->stream));
+    int nz = 1;
+    SWE_CHECK(c, hipMemcpyAsync(&nz, c->bz, sizeof(int), hipMemcpyDeviceToHost, c->stream));
     SWE_CHECK(c, hipStreamSynchronize(c->stream));
+    c->b_zero = nz == 0;
     return SWE_OK;
 }
 
@@ -378,6 +390,7 @@ int swe_hip_create(const swe_grid_desc *d, int device, swe_hip_ctx **out) {
     if (hipMalloc(&c->flag, sizeof(int) * swe::kStatusWords) != hipSuccess)
         return bad(SWE_ERR_OOM, "status words");
     if (hipMalloc(&c->counter, sizeof(long long)) != hipSuccess) return bad(SWE_ERR_OOM, "ctr");
+    if (hipMalloc(&c->bz, sizeof(int)) != hipSuccess) return bad(SWE_ERR_OOM, "bottom word");
     for (double **a : arrays)
         if (swe::launch_fill_double(*a, 0.0, (long)c->n_total, c->stream) != hipSuccess)
             return bad(SWE_ERR_HIP, "fill");
@@ -405,6 +418,7 @@ void swe_hip_destroy(swe_hip_ctx *c) {
     (void)hipFree(c->partial);
     (void)hipFree(c->flag);
     (void)hipFree(c->counter);
+    (void)hipFree(c->bz);
     (void)hipFree(c->hist);
     if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
     delete c;
@@ -575,6 +589,17 @@ int swe_hip_set_tile_rows(swe_hip_ctx *c, int tile_rows) {
 }
 
 int swe_hip_get_tile_rows(const swe_hip_ctx *c) { return c ? c->stage_rows[0] : -1; }
+
+int swe_hip_set_flat_bottom(swe_hip_ctx *c, int mode) {
+    if (!c || (mode != 0 && mode != 1)) return SWE_ERR_INVALID;
+    c->flat_bottom_mode = mode;
+    return SWE_OK;
+}
+
+int swe_hip_get_flat_bottom(const swe_hip_ctx *c) {
+    if (!c) return -1;
+    return flat_bottom(c) ? 1 : 0;
+}
 
 int swe_hip_set_stage_tile_rows(swe_hip_ctx *c, int stage, int tile_rows) {
     if (!c || stage < 1 || stage > 3 || tile_rows < 1) return SWE_ERR_INVALID;

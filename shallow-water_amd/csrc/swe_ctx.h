@@ -26,6 +26,12 @@ struct swe_hip_ctx {
     double *Unext[3] = {nullptr, nullptr, nullptr};
     int fused_rows = 0;
     double *B = nullptr;
+    // the bottom is +0.0 in every cell, ghosts included (checked at every upload); with
+    // flat_bottom_mode on (the default) the stage launches then take the kernels that do not
+    // read it
+    bool b_zero = false;
+    int flat_bottom_mode = 1;
+    int *bz = nullptr;  // device word of that check
     double *scal = nullptr;      // [0] dt, [1] max speed
     double *partial = nullptr;   // per-block maxima
     int partial_cap = 0;
@@ -47,6 +53,7 @@ struct swe_hip_ctx {
     hipGraphExec_t graph = nullptr;
     int graph_mode = -1;
     int graph_rows[3] = {-1, -1, -1};
+    int graph_flat = -1;
     int graph_precision = -1;
     int graph_scheme = -1;  // scheme * 16 + recon * 4 + riemann of the captured step
     hipStream_t graph_stream = nullptr;

@@ -53,6 +53,8 @@ struct StageArgs {
     int *flag;          // run status words (kStatusWords, below)
     int tile_rows;      // interior rows per wave
     int nstrips;        // 60-column strips across y
+    int flat_b;         // host side: B is +0.0 in every cell, ghosts included (the launch
+                        // then takes the kernels that do not read B)
 };
 
 constexpr int kWave = 64;
@@ -190,6 +192,9 @@ hipError_t launch_batch_finalize(const BatchInst *inst, int count, unsigned long
 hipError_t launch_set_scalar(double *dst, double v, hipStream_t stream);
 hipError_t launch_fill_int(int *dst, int v, int n, hipStream_t stream);
 hipError_t launch_fill_double(double *dst, double v, long n, hipStream_t stream);
+// *flag |= 1 when some a[0..n) is not +0.0 (a bit pattern other than all zeros); the caller
+// clears *flag first.
+hipError_t launch_nonzero_bits(const double *a, long n, int *flag, hipStream_t stream);
 
 // ---- discrete adjoint ------------------------------------------------------------------
 // Arguments of one adjoint stage launch (stage s of a step, reversed).

@@ -47,7 +47,7 @@ struct V3 {
 // of face r+1/2.  The caller rotates the register names, so nothing is copied.  (Loading
 // row r+3 a row earlier instead holds a fourth row in registers: 148 VGPRs against 140,
 // and the step measured ~1.5% slower.)
-template <int STAGE, int BC, bool SPEED, bool FAST>
+template <int STAGE, int BC, bool SPEED, bool FAST, bool FLATB = false>
 struct March {
     const StageArgs &A;
     long long base;  // element offset of this grid in the arrays (batched instances)
@@ -72,7 +72,9 @@ struct March {
         c.h = ld(row_ptr(A.in_h, ro), jc_bytes);
         c.hu = ld(row_ptr(A.in_hu, ro), jc_bytes);
         c.hv = ld(row_ptr(A.in_hv, ro), jc_bytes);
-        c.b = ld(row_ptr(A.B, ro), jc_bytes);
+        // FLATB: the bottom is +0.0 everywhere, so it is not read (the same values: every
+        // expression below sees b = +0.0 as it would after the load)
+        c.b = FLATB ? 0.0 : ld(row_ptr(A.B, ro), jc_bytes);
         return c;
     }
 
@@ -85,8 +87,9 @@ struct March {
         }
         c.eta = c.h + c.b;
         const unsigned long long exec = __builtin_amdgcn_read_exec();
-        split = split && (split_ok_lanes(c.h) & split_ok_lanes(c.hu) & split_ok_lanes(c.hv) &
-                          split_ok_lanes(c.b) & exec) == exec;
+        unsigned long long ok = split_ok_lanes(c.h) & split_ok_lanes(c.hu) & split_ok_lanes(c.hv);
+        if (!FLATB) ok &= split_ok_lanes(c.b);
+        split = split && (ok & exec) == exec;
     }
 
     __device__ __forceinline__ Cell load(int r) {
@@ -159,7 +162,7 @@ struct March {
         const double pe = from_next_z(a.eta - sy_e);
         const double pu = from_next_z(a.hu - sy_u);
         const double pv = from_next_z(a.hv - sy_v);
-        const double pb = from_next_z(a.b);
+        const double pb = FLATB ? 0.0 : from_next_z(a.b);
         // normal momentum is hv, tangential hu
         return face_flux<FAST, SPLIT>(half_g, g, a.eta + sy_e, a.hv + sy_v, a.hu + sy_u, a.b,
                                       pe, pv, pu, pb, kLastLanes);
@@ -259,8 +262,8 @@ struct March {
 // UNROLL: the loop is unrolled 6x so that the 3-row window and the 2-deep slope,
 // difference and flux pairs rotate by renaming; without it they rotate by copies (the
 // exact mode's fallback march, cold code).
-template <bool SPLIT_ONLY, bool UNROLL, int STAGE, int BC, bool SPEED, bool FAST>
-__device__ __forceinline__ int march(March<STAGE, BC, SPEED, FAST> &m, int r0, int r1) {
+template <bool SPLIT_ONLY, bool UNROLL, int STAGE, int BC, bool SPEED, bool FAST, bool FLATB>
+__device__ __forceinline__ int march(March<STAGE, BC, SPEED, FAST, FLATB> &m, int r0, int r1) {
     const Cell P2 = m.load(r0 - 2);
     const Cell P1 = m.load(r0 - 1);
     Cell C0 = m.load(r0), C1 = m.load(r0 + 1), C2;
@@ -324,7 +327,7 @@ __device__ __forceinline__ int march(March<STAGE, BC, SPEED, FAST> &m, int r0, i
 // In the exact mode the split-only march runs first; if the range guard fails in a row,
 // the fallback march restarts from that row with its own prologue (the rows before it are
 // stored, and the faces it recomputes have the same values in either form).
-template <int STAGE, int BC, bool SPEED, bool FAST>
+template <int STAGE, int BC, bool SPEED, bool FAST, bool FLATB = false>
 __device__ __forceinline__ void run_rows(const GridDev &G, const StageArgs &A, long long base,
                                          double dt, int col0, int own_lo, int own_hi, int r0,
                                          int r1, int lane, double &speed, int &bad) {
@@ -334,7 +337,7 @@ __device__ __forceinline__ void run_rows(const GridDev &G, const StageArgs &A, l
     const int jl = j < ng - 2 ? ng - 2 : (j < ng + ny + 2 ? j : ng + ny + 1);
     const int jc = mirror<BC>(jl, ng, ng + ny, gcol);
 
-    March<STAGE, BC, SPEED, FAST> m{A, base, G.stride, ng, nx, ng + nx + 1, r1,
+    March<STAGE, BC, SPEED, FAST, FLATB> m{A, base, G.stride, ng, nx, ng + nx + 1, r1,
                                     (unsigned)jl * 8u, (unsigned)jc * 8u, gcol,
                                     lane >= own_lo && lane < own_hi && j >= ng && j < ng + ny,
                                     G.half_g, G.g, G.inv_dx, G.inv_dy, dt, speed, bad,
@@ -351,12 +354,12 @@ __device__ __forceinline__ void run_rows(const GridDev &G, const StageArgs &A, l
 
 // One wave's tile of a stage launch: strip `strip` (60 output columns) of rows
 // [r0, min(r0 + tile_rows, ng + nx)).
-template <int STAGE, int BC, bool SPEED, bool FAST>
+template <int STAGE, int BC, bool SPEED, bool FAST, bool FLATB = false>
 __device__ __forceinline__ void run_tile(const GridDev &G, const StageArgs &A, long long base,
                                          double dt, int strip, int r0, int lane, double &speed,
                                          int &bad) {
     const int r1 = (r0 + A.tile_rows < G.ng + G.nx) ? r0 + A.tile_rows : G.ng + G.nx;
-    run_rows<STAGE, BC, SPEED, FAST>(G, A, base, dt, G.ng - 2 + strip * kStripCols, 2,
+    run_rows<STAGE, BC, SPEED, FAST, FLATB>(G, A, base, dt, G.ng - 2 + strip * kStripCols, 2,
                                      kWave - 2, r0, r1, lane, speed, bad);
 }
 
@@ -388,7 +391,7 @@ __device__ __forceinline__ int xcd_tile(int b, int total) {
 #define SWE_BATCH_XCD_CHUNK 16
 #endif
 
-template <int STAGE, int BC, bool SPEED, bool FAST>
+template <int STAGE, int BC, bool SPEED, bool FAST, bool FLATB>
 __global__ __launch_bounds__(kWave * kStageBlockWaves)
 __attribute__((amdgpu_waves_per_eu(SWE_STAGE_WAVES_PER_EU)))
 void stage_kernel(const GridDev G, const StageArgs A) {
@@ -416,8 +419,8 @@ void stage_kernel(const GridDev G, const StageArgs A) {
     }
 
     if (strip < A.nstrips)
-        run_tile<STAGE, BC, SPEED, FAST>(G, A, 0, dt, strip, G.ng + seg * A.tile_rows, lane,
-                                         speed, bad);
+        run_tile<STAGE, BC, SPEED, FAST, FLATB>(G, A, 0, dt, strip, G.ng + seg * A.tile_rows,
+                                                lane, speed, bad);
 
     if (STAGE != 0) {
         if (__any(bad)) {
@@ -744,28 +747,37 @@ __global__ void fill_int_kernel(int *dst, int v, int n) {
     if (t < n) dst[t] = v;
 }
 
+__global__ __launch_bounds__(256) void nonzero_bits_kernel(const double *a, long n,
+                                                           int *flag) {
+    bool nz = false;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (long)gridDim.x * blockDim.x)
+        nz = nz || __double_as_longlong(a[t]) != 0;
+    if (wave_any(nz) && (threadIdx.x & (kWave - 1)) == 0) atomicOr(flag, 1);
+}
+
 __global__ void fill_double_kernel(double *dst, double v, long n) {
     const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t < n) dst[t] = v;
 }
 
-template <int STAGE, int BC, bool SPEED, bool FAST>
+template <int STAGE, int BC, bool SPEED, bool FAST, bool FLATB>
 hipError_t launch_one(const GridDev &G, const StageArgs &A, hipStream_t stream, dim3 grid) {
-    hipLaunchKernelGGL((stage_kernel<STAGE, BC, SPEED, FAST>), grid, dim3(kWave * kStageBlockWaves), 0,
-                       stream, G, A);
+    hipLaunchKernelGGL((stage_kernel<STAGE, BC, SPEED, FAST, FLATB>), grid,
+                       dim3(kWave * kStageBlockWaves), 0, stream, G, A);
     return hipGetLastError();
 }
 
-template <int BC, bool FAST>
+template <int BC, bool FAST, bool FLATB = false>
 hipError_t launch_bc(int stage, bool speed, const GridDev &G, const StageArgs &A,
                      hipStream_t stream, dim3 grid) {
     switch (stage) {
-        case 0: return launch_one<0, BC, false, FAST>(G, A, stream, grid);
-        case 1: return launch_one<1, BC, false, FAST>(G, A, stream, grid);
-        case 2: return launch_one<2, BC, false, FAST>(G, A, stream, grid);
+        case 0: return launch_one<0, BC, false, FAST, FLATB>(G, A, stream, grid);
+        case 1: return launch_one<1, BC, false, FAST, FLATB>(G, A, stream, grid);
+        case 2: return launch_one<2, BC, false, FAST, FLATB>(G, A, stream, grid);
         case 3:
-            return speed ? launch_one<3, BC, true, FAST>(G, A, stream, grid)
-                         : launch_one<3, BC, false, FAST>(G, A, stream, grid);
+            return speed ? launch_one<3, BC, true, FAST, FLATB>(G, A, stream, grid)
+                         : launch_one<3, BC, false, FAST, FLATB>(G, A, stream, grid);
         default: return hipErrorInvalidValue;
     }
 }
@@ -786,6 +798,10 @@ hipError_t launch_stage(int stage, int bc, bool speed, bool fast, const GridDev 
     if (fast)
         return bc == 0 ? launch_bc<0, true>(stage, speed, G, A, stream, grid)
                        : launch_bc<1, true>(stage, speed, G, A, stream, grid);
+    // a bottom that is +0.0 everywhere: the exact-mode kernels that do not read it
+    if (A.flat_b)
+        return bc == 0 ? launch_bc<0, false, true>(stage, speed, G, A, stream, grid)
+                       : launch_bc<1, false, true>(stage, speed, G, A, stream, grid);
     return bc == 0 ? launch_bc<0, false>(stage, speed, G, A, stream, grid)
                    : launch_bc<1, false>(stage, speed, G, A, stream, grid);
 }
@@ -878,6 +894,11 @@ hipError_t launch_set_scalar(double *dst, double v, hipStream_t stream) {
 
 hipError_t launch_fill_int(int *dst, int v, int n, hipStream_t stream) {
     hipLaunchKernelGGL(fill_int_kernel, dim3((n + 255) / 256), dim3(256), 0, stream, dst, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_nonzero_bits(const double *a, long n, int *flag, hipStream_t stream) {
+    hipLaunchKernelGGL(nonzero_bits_kernel, dim3(1024), dim3(256), 0, stream, a, n, flag);
     return hipGetLastError();
 }
 

@@ -88,6 +88,8 @@ _SIGS = {
     "swe_hip_get_tile_rows": (C.c_int, [C.c_void_p]),
     "swe_hip_set_stage_tile_rows": (C.c_int, [C.c_void_p, C.c_int, C.c_int]),
     "swe_hip_get_stage_tile_rows": (C.c_int, [C.c_void_p, C.c_int]),
+    "swe_hip_set_flat_bottom": (C.c_int, [C.c_void_p, C.c_int]),
+    "swe_hip_get_flat_bottom": (C.c_int, [C.c_void_p]),
     "swe_hip_set_precision": (C.c_int, [C.c_void_p, C.c_int]),
     "swe_hip_get_precision": (C.c_int, [C.c_void_p]),
     "swe_hip_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
@@ -296,6 +298,16 @@ class HipSolver:
     @tile_rows.setter
     def tile_rows(self, v):
         self._check(lib().swe_hip_set_tile_rows(self._ctx, int(v)))
+
+    @property
+    def flat_bottom(self):
+        """True when the stage launches skip B (a bottom of +0.0 everywhere, mode on)."""
+        return lib().swe_hip_get_flat_bottom(self._ctx) == 1
+
+    def set_flat_bottom(self, enabled):
+        """Mode of the flat-bottom kernels: True (default) uses them when B is +0.0
+        everywhere, False always reads B."""
+        self._check(lib().swe_hip_set_flat_bottom(self._ctx, 1 if enabled else 0))
 
     @property
     def stage_tile_rows(self):

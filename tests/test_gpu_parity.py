@@ -172,25 +172,32 @@ def test_adaptive_vs_oracle(hip, bc):
         assert_bitwise(g_, w_, f"adaptive bc={bc} {comp}")
 
 
+@pytest.mark.parametrize("flat", [False, True], ids=["bathy", "flat_bottom"])
 @pytest.mark.parametrize("case", ["tiny_values", "huge_values", "denormals"])
-def test_split_guard_fallback(hip, case):
+def test_split_guard_fallback(hip, case, flat):
     """Cells outside [2^-240, 2^190] (and not 0) switch their waves to the full IEEE division
-    and sqrt sequences; results stay bitwise equal either way."""
-    c = W.random_state(90, 140, seed=31)
+    and sqrt sequences; results stay bitwise equal either way (flat: B = +0.0 everywhere,
+    through the kernels that do not read it)."""
+    c = W.random_state(90, 140, seed=31, bathy=not flat)
     h, hu, hv, B = c["h"], c["hu"], c["hv"], c["B"]
     if case == "tiny_values":
         hu[10:14, 20:70] = 1e-300
         hv[40:42, 100:130] = -3e-250
-        B[60:63, 5:15] = 1e-290
+        if not flat:
+            B[60:63, 5:15] = 1e-290
     elif case == "huge_values":
         hv[30:33, 50:60] = 1e60   # above 2^190 (~1.6e57)
-        B[70, 70:90] = 1e58
+        if not flat:
+            B[70, 70:90] = 1e58
+        else:
+            h[70, 70:90] = 1e58
     else:
         hu[20:25, 80:90] = 5e-320
         hv[50:52, 10:40] = -1e-310
     gr, h0, hu0, hv0, B0 = oracle_state(c)
     want = oracle.rhs(gr, h0.copy(), hu0.copy(), hv0.copy(), B0)
     with gpu_solver(hip, c) as s:
+        assert s.flat_bottom == flat
         got = s.rhs()
         dt, _ = s.stable_dt(0.2)
         s.step(dt, 2)
@@ -242,6 +249,93 @@ def test_stage_tile_rows_invariant(hip, rows):
     assert np.array_equal(hists[0][0], hists[1][0])
     for a, b in zip(hists[0][1], hists[1][1]):
         assert_bitwise(a, b, f"adaptive, stage rows {rows} vs 8")
+
+
+FLAT_CASES = [
+    ("random_37x70", lambda: W.random_state(37, 70, seed=31, bathy=False)),
+    ("random_70x37_wd", lambda: W.random_state(70, 37, seed=32, bathy=False, wet_dry=True)),
+    ("random_41x129_transm", lambda: W.random_state(41, 129, seed=33, bathy=False, bc=1)),
+    ("random_33x33_wd_transm", lambda: W.random_state(33, 33, seed=34, bathy=False,
+                                                      wet_dry=True, bc=1)),
+    ("dam_break_97", lambda: W.dam_break(97)),
+    ("gauss_pulse_64", lambda: W.gauss_pulse(64)),
+]
+
+
+def _gpu_rc(hip, fn):
+    """0 when fn() returns, 1 when it raises the non-finite stop (SWE_ERR_NONFINITE)."""
+    try:
+        fn()
+        return 0
+    except hip.SweError as e:
+        assert e.code == 3, e
+        return 1
+
+
+@pytest.mark.parametrize("name,make", FLAT_CASES, ids=[c[0] for c in FLAT_CASES])
+def test_flat_bottom_kernels(hip, name, make):
+    """A bottom of +0.0 everywhere: the stage kernels that do not read B give the oracle's
+    bits (right-hand side, fixed and adaptive steps, and the arrays a non-finite stop
+    leaves) and those of the kernels that do."""
+    c = make()
+    gr, h, hu, hv, B = oracle_state(c)
+    want_rhs = oracle.rhs(gr, h, hu, hv, B)
+    dt, _ = oracle.stable_dt(gr, h, hu, hv, 0.4)
+    ha, hua, hva = h.copy(), hu.copy(), hv.copy()
+    rc, _, _ = oracle.run(gr, h, hu, hv, B, 5, dt=dt)
+    rca, _, want_hist = oracle.run(gr, ha, hua, hva, B, 4, cfl=0.4)
+    outs = []
+    for flat in (True, False):
+        with gpu_solver(hip, c) as s:
+            s.set_flat_bottom(flat)
+            assert s.flat_bottom == flat
+            rhs = s.rhs()
+            assert _gpu_rc(hip, lambda: s.step(dt, 5)) == rc, f"{name} stop"
+            got = s.download()
+        with gpu_solver(hip, c) as s:
+            s.set_flat_bottom(flat)
+            assert _gpu_rc(hip, lambda: s.step_adaptive(0.4, 4)) == rca, f"{name} stop"
+            hist = s.last_dt_hist
+            got_a = s.download()
+        for comp, g_, w_ in zip(("h", "hu", "hv"), rhs, want_rhs):
+            assert_bitwise(g_, w_, f"{name} flat={flat} rhs {comp}")
+        for comp, g_, w_ in zip(("h", "hu", "hv"), got, (h, hu, hv)):
+            assert_bitwise(g_, w_, f"{name} flat={flat} 5 steps {comp}", nan_any=True)
+        if rca == 0:
+            assert np.array_equal(bits(hist), bits(np.asarray(want_hist))), f"{name} dts"
+        for comp, g_, w_ in zip(("h", "hu", "hv"), got_a, (ha, hua, hva)):
+            assert_bitwise(g_, w_, f"{name} flat={flat} adaptive {comp}", nan_any=True)
+        outs.append(got)
+    for a, b in zip(*outs):
+        assert_bitwise(a, b, f"{name} flat vs general", nan_any=True)
+
+
+def test_flat_bottom_detection(hip):
+    """Only a bottom whose every bit is zero (after the ghost fill) takes the kernels that do
+    not read it: -0.0 or one non-zero cell does not; a new upload re-decides, and the step
+    graph follows."""
+    c = W.random_state(45, 62, seed=35, bathy=False)
+    gr, h, hu, hv, B = oracle_state(c)
+    dt, _ = oracle.stable_dt(gr, h, hu, hv, 0.4)
+    bumpy = c["B"].copy()
+    bumpy[c["ng"] + 20, c["ng"] + 31] = 1e-3
+    negzero = np.full_like(c["B"], -0.0)
+    ghost_only = c["B"].copy()
+    ghost_only[0, 5] = 0.25  # a ghost value: the upload's ghost fill overwrites it
+    with gpu_solver(hip, c) as s:
+        assert s.flat_bottom
+        for Bx, flat in ((bumpy, False), (negzero, False), (ghost_only, True), (c["B"], True),
+                         (bumpy, False)):
+            s.upload(c["h"], c["hu"], c["hv"], Bx)
+            assert s.flat_bottom == flat
+            s.step(dt, 3)
+            got = s.download()
+            Bo = Bx.copy()
+            oracle.apply_bc(gr, Bo)
+            h2, hu2, hv2 = (c[k].copy() for k in ("h", "hu", "hv"))
+            oracle.run(gr, h2, hu2, hv2, Bo, 3, dt=dt)
+            for comp, g_, w_ in zip(("h", "hu", "hv"), got, (h2, hu2, hv2)):
+                assert_bitwise(g_, w_, f"flat={flat} {comp}")
 
 
 def test_graph_and_direct_launch_agree(hip):
