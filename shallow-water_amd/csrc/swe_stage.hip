@@ -129,7 +129,11 @@ struct March {
         else
             fy = y_faces<false>(a);
         finish(n, gr);
-        if (SPLIT_ONLY && !split) return false;
+        // The unlikely marks below (guard failures, the fallback march, floor and non-finite
+        // votes) keep those paths out of the hot loop's code: +0.6-0.8% on the step, the
+        // same on the ensemble; marking the face bodies' own branches too moved stage 2 by
+        // up to +4% on bathymetry.
+        if (SPLIT_ONLY && __builtin_expect(!split, 0)) return false;
         d3 = V3{n.eta - b.eta, n.hu - b.hu, n.hv - b.hv};
         if (SPLIT_ONLY || (!FAST && split))
             fx = x_face<true>(a, b, sa, sb, d2, d3);
@@ -219,14 +223,16 @@ struct March {
             if (owner) {
                 // enforce_positivity (omp_kernels.hpp:317-325); the selects run only when some
                 // lane of the wave is below the floor (same values)
-                if (wave_any(hN < kFloor)) {
+                if (__builtin_expect(wave_any(hN < kFloor), 0)) {
                     if (hN < kFloor) {
                         hN = kFloor;
                         huN = 0.0;
                         hvN = 0.0;
                     }
                 }
-                if (wave_any3(!isfinite(hN), !isfinite(huN), !isfinite(hvN))) bad = 1;
+                if (__builtin_expect(wave_any3(!isfinite(hN), !isfinite(huN), !isfinite(hvN)),
+                                     0))
+                    bad = 1;
                 if (SPEED) speed = cell_speed(g, hN, huN, hvN, speed);
                 pin_lane_offset(j_bytes, r);
                 st(row_ptr(A.out_h, ro), j_bytes, hN);
@@ -267,7 +273,7 @@ __device__ __forceinline__ int march(March<STAGE, BC, SPEED, FAST, FLATB> &m, in
     const Cell P2 = m.load(r0 - 2);
     const Cell P1 = m.load(r0 - 1);
     Cell C0 = m.load(r0), C1 = m.load(r0 + 1), C2;
-    if (SPLIT_ONLY && !m.split) return r0;
+    if (SPLIT_ONLY && __builtin_expect(!m.split, 0)) return r0;
 
     // x differences d_{r0-3/2}, d_{r0-1/2}, d_{r0+1/2}
     const V3 dm{P1.eta - P2.eta, P1.hu - P2.hu, P1.hv - P2.hv};
@@ -344,7 +350,7 @@ __device__ __forceinline__ void run_rows(const GridDev &G, const StageArgs &A, l
                                     G.split_ok != 0};
     int r = r0;
     if (!FAST && G.split_ok) r = march<true, true>(m, r0, r1);
-    if (r < r1) {
+    if (__builtin_expect(r < r1, 0)) {
         m.split = G.split_ok != 0;
         march<false, true>(m, r, r1);
     }
