@@ -478,7 +478,7 @@ void stage_kernel(const GridDev G, const StageArgs A) {
 constexpr int kFusedCols = kWave - 12;
 constexpr int kFusedSkipped = 8;
 
-template <int BC, bool SPEED, bool FAST>
+template <int BC, bool SPEED, bool FAST, bool FLATB>
 __global__ __launch_bounds__(kWave * kStageBlockWaves)
 __attribute__((amdgpu_waves_per_eu(SWE_STAGE_WAVES_PER_EU)))
 void fused_step_kernel(const GridDev G, const StageArgs A1, const StageArgs A2,
@@ -506,16 +506,16 @@ void fused_step_kernel(const GridDev G, const StageArgs A1, const StageArgs A2,
         const int r0 = ng + seg * A1.tile_rows;
         const int r1 = (r0 + A1.tile_rows < ie) ? r0 + A1.tile_rows : ie;
         double unused = 0.0;
-        run_rows<1, BC, false, FAST>(G, A1, 0, dt, col0, 2, kWave - 2,
-                                     r0 - 4 > ng ? r0 - 4 : ng, r1 + 4 < ie ? r1 + 4 : ie,
-                                     lane, unused, bad1);
+        run_rows<1, BC, false, FAST, FLATB>(G, A1, 0, dt, col0, 2, kWave - 2,
+                                            r0 - 4 > ng ? r0 - 4 : ng,
+                                            r1 + 4 < ie ? r1 + 4 : ie, lane, unused, bad1);
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-        run_rows<2, BC, false, FAST>(G, A2, 0, dt, col0, 4, kWave - 4,
-                                     r0 - 2 > ng ? r0 - 2 : ng, r1 + 2 < ie ? r1 + 2 : ie,
-                                     lane, unused, bad2);
+        run_rows<2, BC, false, FAST, FLATB>(G, A2, 0, dt, col0, 4, kWave - 4,
+                                            r0 - 2 > ng ? r0 - 2 : ng,
+                                            r1 + 2 < ie ? r1 + 2 : ie, lane, unused, bad2);
         __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "workgroup");
-        run_rows<3, BC, SPEED, FAST>(G, A3, 0, dt, col0, 6, kWave - 6, r0, r1, lane, speed,
-                                     bad3);
+        run_rows<3, BC, SPEED, FAST, FLATB>(G, A3, 0, dt, col0, 6, kWave - 6, r0, r1, lane,
+                                            speed, bad3);
     }
     const int bits = (__any(bad1) ? 1 : 0) | (__any(bad2) ? 2 : 0) | (__any(bad3) ? 4 : 0);
     if (bits != 0 && lane == 0) atomicOr(status + 1 + parity, bits);
@@ -828,16 +828,28 @@ hipError_t launch_fused_step(int bc, bool speed, bool fast, const GridDev &G,
     const dim3 grid((unsigned)fused_step_blocks(G, A1.tile_rows), 1, 1);
     if (grid_out) *grid_out = grid;
     const dim3 block(kWave * kStageBlockWaves);
-#define SWE_FUSED(BCV, SP, FA)                                                              \
-    hipLaunchKernelGGL((fused_step_kernel<BCV, SP, FA>), grid, block, 0, stream, G, A1, A2, A3, \
-                       parity)
+#define SWE_FUSED(BCV, SP, FA, FB)                                                   \
+    hipLaunchKernelGGL((fused_step_kernel<BCV, SP, FA, FB>), grid, block, 0, stream, G, A1, \
+                       A2, A3, parity)
+#define SWE_FUSED_SP(BCV, FA, FB)            \
+    do {                                     \
+        if (speed)                           \
+            SWE_FUSED(BCV, true, FA, FB);    \
+        else                                 \
+            SWE_FUSED(BCV, false, FA, FB);   \
+    } while (0)
+    // the exact mode with a bottom of +0.0 everywhere: the marches that do not read B
+    const bool fb = !fast && A1.flat_b;
     if (bc == 0) {
-        if (fast) { if (speed) SWE_FUSED(0, true, true); else SWE_FUSED(0, false, true); }
-        else { if (speed) SWE_FUSED(0, true, false); else SWE_FUSED(0, false, false); }
+        if (fast) SWE_FUSED_SP(0, true, false);
+        else if (fb) SWE_FUSED_SP(0, false, true);
+        else SWE_FUSED_SP(0, false, false);
     } else {
-        if (fast) { if (speed) SWE_FUSED(1, true, true); else SWE_FUSED(1, false, true); }
-        else { if (speed) SWE_FUSED(1, true, false); else SWE_FUSED(1, false, false); }
+        if (fast) SWE_FUSED_SP(1, true, false);
+        else if (fb) SWE_FUSED_SP(1, false, true);
+        else SWE_FUSED_SP(1, false, false);
     }
+#undef SWE_FUSED_SP
 #undef SWE_FUSED
     return hipGetLastError();
 }

@@ -41,17 +41,41 @@ def hipcc() -> str:
     raise RuntimeError("hipcc not found")
 
 
-def build(verbose: bool = False, extra=None) -> str:
-    srcs = [os.path.join(CSRC, s) for s in SOURCES]
-    tmp = LIB + ".tmp"
-    cmd = [hipcc(), *HIPCC_FLAGS, *(extra or []), "-I", os.path.join(REPO, "include"), "-o", tmp,
-           *srcs]
+def _run(cmd, verbose):
     if verbose:
         print(" ".join(cmd), flush=True)
-    res = subprocess.run(cmd, capture_output=True, text=True)
-    if res.returncode != 0:
-        sys.stderr.write(res.stdout + res.stderr)
-        raise RuntimeError(f"hipcc failed ({res.returncode})")
+    return subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+
+
+def build(verbose: bool = False, extra=None) -> str:
+    """Compile each source to an object in parallel (the stage kernels' file dominates), then
+    link the shared library."""
+    import tempfile
+
+    inc = ["-I", os.path.join(REPO, "include")]
+    compile_flags = [f for f in HIPCC_FLAGS if f not in ("-shared",)]
+    with tempfile.TemporaryDirectory(prefix="swe_hip_build_") as tmpdir:
+        objs, procs = [], []
+        for src in SOURCES:
+            obj = os.path.join(tmpdir, os.path.splitext(src)[0] + ".o")
+            objs.append(obj)
+            procs.append((src, _run([hipcc(), *compile_flags, *(extra or []), *inc, "-c",
+                                     "-o", obj, os.path.join(CSRC, src)], verbose)))
+        failed = []
+        for src, pr in procs:
+            out, _ = pr.communicate()
+            if pr.returncode != 0:
+                failed.append((src, pr.returncode, out))
+        if failed:
+            for src, rc, out in failed:
+                sys.stderr.write(f"--- {src} ({rc})\n{out}")
+            raise RuntimeError(f"hipcc failed on {', '.join(f[0] for f in failed)}")
+        tmp = LIB + ".tmp"
+        link = _run([hipcc(), *HIPCC_FLAGS, *(extra or []), "-o", tmp, *objs], verbose)
+        out, _ = link.communicate()
+        if link.returncode != 0:
+            sys.stderr.write(out)
+            raise RuntimeError(f"hipcc link failed ({link.returncode})")
     os.replace(tmp, LIB)
     return LIB
 
