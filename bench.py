@@ -426,6 +426,8 @@ def main():
     ap.add_argument("--no-c1", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tile-rows", type=int, default=0)
+    ap.add_argument("--stage-tile-rows", type=str, default="",
+                    help="tile heights of stages 1, 2, 3 (comma-separated)")
     ap.add_argument("--event-steps", type=int, default=20)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=500)
@@ -463,6 +465,8 @@ def main():
     s = swe_hip.HipSolver.from_case(case)
     if args.tile_rows:
         s.tile_rows = args.tile_rows
+    if args.stage_tile_rows:
+        s.stage_tile_rows = [int(x) for x in args.stage_tile_rows.split(",")]
     if args.precision == "fast":
         s.precision = swe_hip.PRECISION_FAST
     s.upload(case["h"], case["hu"], case["hv"], case["B"])
