@@ -545,9 +545,8 @@ struct LdsGather {
 // The ghost rows' x parts folded into the interior rows they mirror (reflecting: ng-1 ->
 // ng, ng-2 -> ng+1; transmissive: both -> ng; likewise at the far wall).
 template <int BC, class S>
-__device__ __forceinline__ A4 gather_folded(const S &X, const GridDev &G, int r) {
+__device__ __forceinline__ A4 fold_ghost_rows(const S &X, const GridDev &G, int r, A4 T) {
     const int ng = G.ng, nx = G.nx;
-    A4 T = gather_row(X, G, r);
     if (r == ng) {
         fold_row<BC>(T, gather_row(X, G, ng - 1));
         if (BC != 0) fold_row<BC>(T, gather_row(X, G, ng - 2));
@@ -558,6 +557,102 @@ __device__ __forceinline__ A4 gather_folded(const S &X, const GridDev &G, int r)
         if (BC != 0) fold_row<BC>(T, gather_row(X, G, ng + nx + 1));
     }
     if (BC == 0 && r == ng + nx - 2) fold_row<BC>(T, gather_row(X, G, ng + nx + 1));
+    return T;
+}
+template <int BC, class S>
+__device__ __forceinline__ A4 gather_folded(const S &X, const GridDev &G, int r) {
+    return fold_ghost_rows<BC>(X, G, r, gather_row(X, G, r));
+}
+
+// ---- rolling x gather (the fused kernel's rows) ----------------------------------------
+// A wave walking rows ra .. rb-1 routes each row's slope adjoint once and keeps the parts
+// of rows r-1, r, r+1 in registers, where gather_row routes all three for every row.  The
+// sums are gather_row's, term by term and in its order, so the bits are the same.  The LDS
+// reads of row r+1's slope inputs are issued before the row's y faces and used after them
+// (issuing the direct parts early as well measured no better).  Interior rows r only need slope
+// rows ng-1 .. ng+nx, which all exist; a face index is clamped into the block's range
+// [F0, F1], which holds every face inside the domain, and a face outside the domain is
+// read there but not used.
+struct SlopeIn {
+    double a[3];  // minus-state adjoints (e, u, v) of face t
+    double m[3];  // plus-state adjoints of face t-1
+    int sel;
+};
+struct SlopeRow {
+    double p[3], s[3], n[3];  // parts routed to rows t-1, t, t+1 (e, u, v)
+};
+struct DirectIn {
+    double m[4];  // face r, minus side (e, u, v, b)
+    double p[4];  // face r-1, plus side
+};
+__device__ __forceinline__ int clamp_face(int f, int F0, int F1) {
+    return f < F0 ? F0 : (f > F1 ? F1 : f);
+}
+__device__ __forceinline__ SlopeIn slope_in(const LdsGather &X, int F1, int t) {
+    const int fa = clamp_face(t, X.F0, F1), fb = clamp_face(t - 1, X.F0, F1);
+    SlopeIn I;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        I.a[k] = X.xf(k, fa);
+        I.m[k] = X.xf(4 + k, fb);
+    }
+    I.sel = X.sel(t);
+    return I;
+}
+__device__ __forceinline__ SlopeRow slope_row(const SlopeIn &I, const GridDev &G, int t) {
+    double ge = 0.0, gu = 0.0, gv = 0.0;
+    if (t >= G.ng - 1 && t <= G.ng + G.nx - 1) {
+        ge += I.a[0];
+        gu += I.a[1];
+        gv += I.a[2];
+    }
+    if (t - 1 >= G.ng - 1 && t - 1 <= G.ng + G.nx - 1) {
+        ge -= I.m[0];
+        gu -= I.m[1];
+        gv -= I.m[2];
+    }
+    SlopeRow S;
+    slope_parts(I.sel & 3, ge, S.p[0], S.s[0], S.n[0]);
+    slope_parts((I.sel >> 2) & 3, gu, S.p[1], S.s[1], S.n[1]);
+    slope_parts((I.sel >> 4) & 3, gv, S.p[2], S.s[2], S.n[2]);
+    return S;
+}
+__device__ __forceinline__ DirectIn direct_in(const LdsGather &X, int F1, int r) {
+    const int fa = clamp_face(r, X.F0, F1), fb = clamp_face(r - 1, X.F0, F1);
+    DirectIn D;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) {
+        D.m[k] = X.xf(k, fa);
+        D.p[k] = X.xf(4 + k, fb);
+    }
+    return D;
+}
+// gather_row(X, G, r) from the direct parts and the slope rows r-1 (Sm), r (S0), r+1 (Sp).
+__device__ __forceinline__ A4 gather_rolled(const GridDev &G, int r, const DirectIn &D,
+                                           const SlopeRow &Sm, const SlopeRow &S0,
+                                           const SlopeRow &Sp) {
+    A4 T{0.0, 0.0, 0.0, 0.0};
+    if (r >= G.ng - 1 && r <= G.ng + G.nx - 1) {
+        T.e += D.m[0];
+        T.u += D.m[1];
+        T.v += D.m[2];
+        T.b += D.m[3];
+    }
+    if (r - 1 >= G.ng - 1 && r - 1 <= G.ng + G.nx - 1) {
+        T.e += D.p[0];
+        T.u += D.p[1];
+        T.v += D.p[2];
+        T.b += D.p[3];
+    }
+    T.e += Sm.n[0];
+    T.u += Sm.n[1];
+    T.v += Sm.n[2];
+    T.e += S0.s[0];
+    T.u += S0.s[1];
+    T.v += S0.s[2];
+    T.e += Sp.p[0];
+    T.u += Sp.p[1];
+    T.v += Sp.p[2];
     return T;
 }
 
@@ -809,13 +904,22 @@ void adjoint_stage_kernel(const GridDev G, const AdjArgs A) {
             bool split = G.split_ok != 0;
             bool fl;
             PCell P0, P1, P2, P3;
-            D3 S0, S1;
+            D3 S0, S1, L0, L1, L2;
             int sel0, sel1;
             {
-                const PCell Pm = w.load(fa - 1, fl);
-                P0 = w.load(fa, fl);
-                P1 = w.load(fa + 1, fl);
-                P2 = w.load(fa + 2, fl);
+                // the four rows and two lamW rows are all in flight before the first is used
+                bool gm, g0, g1, g2;
+                PCell Pm = w.load_raw(fa - 1, gm);
+                P0 = w.load_raw(fa, g0);
+                P1 = w.load_raw(fa + 1, g1);
+                P2 = w.load_raw(fa + 2, g2);
+                const auto L0r = w.load_lam_raw(fa), L1r = w.load_lam_raw(fa + 1);
+                w.finish(Pm, gm, fl);
+                w.finish(P0, g0, fl);
+                w.finish(P1, g1, fl);
+                w.finish(P2, g2, fl);
+                L0 = w.finish_lam(L0r);
+                L1 = w.finish_lam(L1r);
                 split = split && Lanes<BC>::all_lanes(
                                     Lanes<BC>::split_lanes(Pm) & Lanes<BC>::split_lanes(P0) &
                                     Lanes<BC>::split_lanes(P1) & Lanes<BC>::split_lanes(P2));
@@ -825,7 +929,6 @@ void adjoint_stage_kernel(const GridDev G, const AdjArgs A) {
                 S0.v = half_minmod(P0.hv - Pm.hv, P1.hv - P0.hv, sc);
                 sel0 = sa | (sb << 2) | (sc << 4);
             }
-            D3 L0 = w.load_lam(fa), L1 = w.load_lam(fa + 1), L2;
 #pragma unroll 1
             for (int f = fa; f < fb; ++f) {
                 // row f+3 and lamW of row f+2: loaded now, finished after the face
@@ -874,34 +977,47 @@ void adjoint_stage_kernel(const GridDev G, const AdjArgs A) {
             }
         }
     }
-    __syncthreads();
-
-    // ---- phase 2: rows ra .. rb-1 of this wave (an equal share of the block's rows)
+    // ---- phase 2: rows ra .. rb-1 of this wave (an equal share of the block's rows); the
+    // first row and its lamW are loaded before the barrier (ra <= Rend <= ng+nx: in the
+    // padded layout even when the share is empty)
     const int ra = R0 + (wv * (Rend - R0)) / W;
     const int rb = R0 + ((wv + 1) * (Rend - R0)) / W;
+    bool gra;
+    PCell P = w.load_raw(ra, gra);
+    const auto Lra = w.load_lam_raw(ra);
+    __syncthreads();
     if (ra >= rb) return;
     const double ky = G.inv_dy * (cv * *A.dt);
     const LdsGather X{lds_faces, lds_sel, F0, lane};
     bool fl, fln;
-    PCell P = w.load(ra, fl);
-    D3 L = w.load_lam(ra);
+    w.finish(P, gra, fl);
+    D3 L = w.finish_lam(Lra);
+    SlopeRow Sm = slope_row(slope_in(X, F1, ra - 1), G, ra - 1);
+    SlopeRow S0 = slope_row(slope_in(X, F1, ra), G, ra);
 #pragma unroll 1
     for (int r = ra; r < rb; ++r) {
-        // the next row, its lamW and this row's read-back values are loaded first and
-        // finished after the row's arithmetic
+        // the next row, its lamW, this row's read-back values and the next slope row's LDS
+        // inputs are loaded first and finished after the row's arithmetic
         const int rn = r + 1 < rb ? r + 1 : r;
         bool grn;
         PCell Pn = w.load_raw(rn, grn);
         const auto Lnr = w.load_lam_raw(rn);
         const EmitIn E = load_emit<STAGE, BC>(A, w, r);
+        const SlopeIn Spi = slope_in(X, F1, r + 1);
         A4 acc = (G.split_ok && Lanes<BC>::all_lanes(Lanes<BC>::split_lanes(P)))
                      ? w.template y_row<true>(G.half_g, G.g, ky, P, L)
                      : w.template y_row<false>(G.half_g, G.g, ky, P, L);
         use_emit<STAGE>(E);
+        const DirectIn Di = direct_in(X, F1, r);
+        const SlopeRow Sp = slope_row(Spi, G, r + 1);
         if (w.owner) {
             A.mask_in[w.row_off(r) + (w.j_bytes >> 3)] = fl ? 1 : 0;
-            emit_cell<STAGE, BC>(A, w, r, L, acc, gather_folded<BC>(X, G, r), E);
+            emit_cell<STAGE, BC>(A, w, r, L, acc,
+                                 fold_ghost_rows<BC>(X, G, r, gather_rolled(G, r, Di, Sm, S0, Sp)),
+                                 E);
         }
+        Sm = S0;
+        S0 = Sp;
         w.finish(Pn, grn, fln);
         P = Pn;
         L = w.finish_lam(Lnr);
