@@ -342,12 +342,6 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
         const unsigned long long exec = __builtin_amdgcn_read_exec();
         split = split && (ok & exec) == exec;
     };
-    auto load = [&](int r) {
-        bool gr;
-        MCell c = load_raw(r, gr);
-        finish(c, gr);
-        return c;
-    };
     // x face between rows a (minus, slope sa) and b (plus, slope sb): normal hu
     auto xface = [&](auto split_tag, const MCell &a, const MSlope &sa, const MCell &b,
                      const MSlope &sb) {
@@ -364,9 +358,17 @@ __global__ __launch_bounds__(256) void modular_stage_kernel(const GridDev G, con
     };
     const MSlope zero{0.0, 0.0, 0.0};
 
-    // prologue: rows r0-2 .. r0+2, the plus side of face r0-1/2
-    const MCell Pm2 = load(r0 - 2), Pm1 = load(r0 - 1);
-    MCell P0 = load(r0), P1 = load(r0 + 1), P2 = load(r0 + 2 <= rmax ? r0 + 2 : rmax);
+    // prologue: rows r0-2 .. r0+2 (all five in flight before the first is used), the plus
+    // side of face r0-1/2
+    bool gm2, gm1, g0, g1, g2;
+    MCell Pm2 = load_raw(r0 - 2, gm2), Pm1 = load_raw(r0 - 1, gm1);
+    MCell P0 = load_raw(r0, g0), P1 = load_raw(r0 + 1, g1);
+    MCell P2 = load_raw(r0 + 2 <= rmax ? r0 + 2 : rmax, g2);
+    finish(Pm2, gm2);
+    finish(Pm1, gm1);
+    finish(P0, g0);
+    finish(P1, g1);
+    finish(P2, g2);
     MSlope S0 = zero, S1 = zero;
     if (RECON == kReconMUSCL) S0 = half_slope_of<false>(Pm1, P0, P1);
     const MSlope Sm1 = RECON == kReconMUSCL ? half_slope_of<false>(Pm2, Pm1, P0) : zero;

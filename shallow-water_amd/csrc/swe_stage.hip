@@ -270,9 +270,15 @@ struct March {
 // exact mode's fallback march, cold code).
 template <bool SPLIT_ONLY, bool UNROLL, int STAGE, int BC, bool SPEED, bool FAST, bool FLATB>
 __device__ __forceinline__ int march(March<STAGE, BC, SPEED, FAST, FLATB> &m, int r0, int r1) {
-    const Cell P2 = m.load(r0 - 2);
-    const Cell P1 = m.load(r0 - 1);
-    Cell C0 = m.load(r0), C1 = m.load(r0 + 1), C2;
+    // the four rows are all in flight before the first is used (one memory round trip)
+    bool g2, g1, g0, gc1;
+    Cell P2 = m.load_raw(r0 - 2, g2);
+    Cell P1 = m.load_raw(r0 - 1, g1);
+    Cell C0 = m.load_raw(r0, g0), C1 = m.load_raw(r0 + 1, gc1), C2;
+    m.finish(P2, g2);
+    m.finish(P1, g1);
+    m.finish(C0, g0);
+    m.finish(C1, gc1);
     if (SPLIT_ONLY && __builtin_expect(!m.split, 0)) return r0;
 
     // x differences d_{r0-3/2}, d_{r0-1/2}, d_{r0+1/2}
