@@ -1075,18 +1075,32 @@ hipError_t launch_adj_one(const GridDev &G, const AdjArgs &A, hipStream_t stream
 }
 
 // Row blocks of the fused kernel.  A block holds at most cap = W * tile_rows rows (its LDS
-// layout); with C blocks resident at once the launch takes ceil(nstrips * nb / C) rounds of
-// a block's time, which grows with its rows plus the 3 extra x faces it recomputes.  The
-// count nb (>= ceil(nx / cap)) minimising rounds x (rows + 3) avoids a nearly empty last
-// round (1024^2 at cap 12: 86 blocks per strip in 4 rounds, the last one 2% full).
-int adj_row_blocks(int nx, int nstrips, int cap, long long resident) {
+// layout) and keeps its slot until the slowest of its W waves is done with each phase: a
+// block of h rows takes about 2 + ceil((h+3)/W) + ceil(h/W) wave-row units (the two
+// prologues, its h+3 x faces, its h rows).  Blocks have floor(nx/nb) or ceil(nx/nb) rows,
+// mixed evenly in dispatch order; with C resident at once, N = nstrips * nb of them take
+// about floor(N/C) rounds of the mean block time plus, if a partial round is left, one
+// block of the longest.  The count nb (>= ceil(nx / cap)) minimising that is taken, the
+// largest of equal ones (below one round they all price as one block: more blocks fill more
+// CUs).  At
+// 1024^2 (18 strips, 512 resident blocks of 4 waves, cap 16) it picks 85 blocks per strip,
+// 81 of them 12 rows high; pricing a block by rows + 3 alone picked 79 of 12-13 rows,
+// whose 13-row blocks leave three of four waves idle in the row phase: 2% slower (84 and
+// 85 against 79, 80 and 82, interleaved runs on one box; the gradient is the same bits for
+// every count).
+int adj_row_blocks(int nx, int nstrips, int W, int cap, long long resident) {
+    auto units = [W](int h) { return 2 + (h + 3 + W - 1) / W + (h + W - 1) / W; };
     const int lo = (nx + cap - 1) / cap;
     int best = lo;
-    long long best_cost = -1;
+    double best_cost = -1.0;
     for (int nb = lo; nb <= nx && nb <= 4 * lo + 64; ++nb) {
-        const long long rounds = ((long long)nstrips * nb + resident - 1) / resident;
-        const long long cost = rounds * ((nx + nb - 1) / nb + 3);
-        if (best_cost < 0 || cost < best_cost) {
+        const int h = nx / nb, big = nx % nb;  // `big` blocks of h + 1 rows, the rest h
+        const double mean =
+            ((double)big * units(h + 1) + (double)(nb - big) * units(h)) / (double)nb;
+        const long long n = (long long)nstrips * nb;
+        const double cost =
+            (double)(n / resident) * mean + (n % resident ? units(big ? h + 1 : h) : 0);
+        if (best_cost < 0.0 || cost <= best_cost + 1e-9) {  // ties: more, shorter blocks
             best_cost = cost;
             best = nb;
         }
@@ -1129,7 +1143,7 @@ hipError_t launch_fused_w(const GridDev &G, const AdjArgs &A0, hipStream_t strea
                                        (int)lds_cap);
     if (e != hipSuccess) return e;
     const long long resident = adj_resident_blocks(fn, kWave * W, lds_cap);
-    const int nb = adj_row_blocks(G.nx, A.nstrips, cap, resident);
+    const int nb = adj_row_blocks(G.nx, A.nstrips, W, cap, resident);
     A.block_rows = (G.nx + nb - 1) / nb;  // <= cap
     const size_t lds = adj_lds_rows_bytes(A.block_rows);
     hipLaunchKernelGGL((adjoint_stage_kernel<STAGE, BC, W>), dim3((unsigned)A.nstrips, (unsigned)nb),
