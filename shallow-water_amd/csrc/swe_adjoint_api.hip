@@ -162,7 +162,10 @@ struct swe_hip_adj {
     int64_t rec_slots = 0;    // records plan: record slots (RecordPlan::slots())
     double *rec = nullptr;    // records plan: RecordPlan::slots() records of 9 arrays
                               // (U^p, U1, U2)
-    hipStream_t stream2 = nullptr;  // records plan: the segment recompute stream
+    hipStream_t stream2 = nullptr;  // the recompute stream (records plan with two segments
+                                    // or more; binomial schedule with overlap buffers)
+    double *ovl[9] = {};  // binomial overlap: a second work state (0..2) and a second pair
+                          // of stage states (U1: 3..5, U2: 6..8); null without overlap
     int per_slot = 3;
     double *W[3] = {nullptr, nullptr, nullptr};
     double *lam[4][3] = {};   // step adjoint, dL/dU2, dL/dU1, U^n accumulator
@@ -489,6 +492,91 @@ int backward_records(swe_hip_adj *a) {
     return SWE_OK;
 }
 
+// Binomial schedule, backward with the recomputation on stream2: the advances and the
+// stage states of each reversed step are issued there, one reversal ahead of the adjoint
+// kernels on the main stream.  The stage states alternate between two pairs (the
+// context's U1/U2 and a second pair), the advances' work state between two buffers:
+//   ready[k]  (stream2)  pair k holds the stage states of the reversal that reads it next;
+//   done[k]   (main)     the last reversal issued with pair k has finished;
+// each work buffer and checkpoint slot remembers the pair of the last reversal that read
+// it, and stream2 waits for that reversal before it overwrites it (an event waited on is
+// its latest record, a reversal at least that late, already issued after its own ready:
+// no cycle).  The adjoint kernels read the same states and run in the same order as on
+// one stream, so the gradient is the same bits.
+int backward_binomial_overlap(swe_hip_adj *a) {
+    swe_hip_ctx *c = a->ctx;
+    hipStream_t s2 = a->stream2;
+    swe::EventSet ev(5);  // 0/1 ready[k], 2/3 done[k], 4 stream hand-over
+    ADJ_CHECK(a, ev.status);
+    const int rows = recompute_rows(c);
+    const State pair[2][2] = {{scratch_stage(a, 1), scratch_stage(a, 2)},
+                              {State{{a->ovl[3], a->ovl[4], a->ovl[5]}},
+                               State{{a->ovl[6], a->ovl[7], a->ovl[8]}}}};
+    const State work[2] = {state_of(a, -1), State{{a->ovl[0], a->ovl[1], a->ovl[2]}}};
+    std::vector<int> slot_reader((size_t)std::max(a->nsnap, 1), -1);
+    int work_reader[2] = {-1, -1};
+    bool issued[2] = {false, false};  // a reversal with pair k has been issued
+    int last_work = 0;  // work buffer of the last advance into work (the forward's: W)
+    int64_t j = 0;                    // reversals issued
+    auto wait_for = [&](int k) -> int {
+        if (k >= 0 && issued[k]) ADJ_CHECK(a, hipStreamWaitEvent(s2, ev[2 + k], 0));
+        return SWE_OK;
+    };
+    // stream2 starts after the main stream's work so far (the forward's checkpoints)
+    ADJ_CHECK(a, hipEventRecord(ev[4], c->stream));
+    ADJ_CHECK(a, hipStreamWaitEvent(s2, ev[4], 0));
+    for (size_t i = a->first_adj; i < a->acts.size(); ++i) {
+        const Action &act = a->acts[i];
+        const int k = (int)(j % 2);  // the pair of the next reversal
+        int rc = SWE_OK;
+        if (act.kind == kAdvance) {
+            const int w = 1 - last_work;  // not the buffer the last reversal from work read
+            rc = wait_for(work_reader[w]);
+            if (!rc) rc = wait_for(k);  // the intermediate stage states use pair k
+            if (!rc && act.dst >= 0) rc = wait_for(slot_reader[act.dst]);
+            if (rc) return rc;
+            work_reader[w] = -1;
+            if (act.dst >= 0) slot_reader[act.dst] = -1;
+            const State src = act.src < 0 ? work[last_work] : state_of(a, act.src);
+            for (int64_t p = act.a; p < act.b && !rc; ++p) {
+                const State in = p == act.a ? src : work[w];
+                const State out =
+                    (p == act.b - 1 && act.dst >= 0) ? state_of(a, act.dst) : work[w];
+                rc = stages_12(a, in, pair[k][0], pair[k][1], p, s2, rows);
+                if (!rc)
+                    ADJ_CHECK(a, swe::launch_stage(3, c->bc, false, false, c->G,
+                                                   fwd_args(a, pair[k][1], in, out, a->dts + p,
+                                                            rows),
+                                                   s2, nullptr));
+            }
+            if (rc) return rc;
+            if (act.dst < 0) last_work = w;
+            continue;
+        }
+        // reversal of step act.a from its start state
+        const State base = act.src < 0 ? work[last_work] : state_of(a, act.src);
+        rc = wait_for(k);
+        if (!rc) rc = stages_12(a, base, pair[k][0], pair[k][1], act.a, s2, rows);
+        if (rc) return rc;
+        ADJ_CHECK(a, hipEventRecord(ev[k], s2));
+        ADJ_CHECK(a, hipStreamWaitEvent(c->stream, ev[k], 0));
+        rc = reverse_with(a, base, pair[k][0], pair[k][1], act.a);
+        if (rc) return rc;
+        ADJ_CHECK(a, hipEventRecord(ev[2 + k], c->stream));
+        issued[k] = true;
+        if (act.src < 0)
+            work_reader[last_work] = k;
+        else
+            slot_reader[act.src] = k;
+        ++j;
+    }
+    ADJ_CHECK(a, hipEventRecord(ev[4], s2));
+    ADJ_CHECK(a, hipStreamWaitEvent(c->stream, ev[4], 0));
+    // the events are released on return: drain both streams' use of them first
+    ADJ_CHECK(a, hipStreamSynchronize(c->stream));
+    return SWE_OK;
+}
+
 // The forward's run status words (swe::kStatusWords): a non-finite stage stopped it there.
 int check_nonfinite(swe_hip_adj *a) {
     int w[swe::kStatusWords] = {};
@@ -528,6 +616,9 @@ int backward_common(swe_hip_adj *a, const double *lh, const double *lhu, const d
     a->state = 2;
     if (a->records) {
         const int rc = backward_records(a);
+        if (rc) return rc;
+    } else if (a->stream2 && a->ovl[0]) {
+        const int rc = backward_binomial_overlap(a);
         if (rc) return rc;
     } else {
         for (size_t i = a->first_adj; i < a->acts.size(); ++i) {
@@ -603,6 +694,23 @@ int adj_setup_binomial(swe_hip_adj *a, int32_t nsnap) {
             a->first_adj = i;
             break;
         }
+    // the overlapped backward's working memory (9 arrays, like the work state and the
+    // context's stage states not counted as stored states); without it the backward runs
+    // the schedule on one stream
+    if (a->nsteps >= 2) {
+        bool ok = hipStreamCreateWithFlags(&a->stream2, hipStreamNonBlocking) == hipSuccess;
+        for (auto &p : a->ovl)
+            ok = ok && hipMalloc(&p, a->n_total * sizeof(double)) == hipSuccess;
+        if (!ok) {
+            (void)hipGetLastError();
+            for (auto &p : a->ovl) {
+                (void)hipFree(p);
+                p = nullptr;
+            }
+            if (a->stream2) (void)hipStreamDestroy(a->stream2);
+            a->stream2 = nullptr;
+        }
+    }
     return SWE_OK;
 }
 
@@ -757,6 +865,7 @@ void swe_hip_adj_destroy(swe_hip_adj *a) {
         (void)hipStreamDestroy(a->stream2);
     }
     for (auto *p : a->W) (void)hipFree(p);
+    for (auto *p : a->ovl) (void)hipFree(p);
     for (auto &l : a->lam)
         for (auto *p : l) (void)hipFree(p);
     (void)hipFree(a->lam_b);
