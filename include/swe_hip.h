@@ -290,7 +290,9 @@ typedef enum {
 } swe_dt_mode;
 
 /* nsnap = number of stored states, U^0 included (>= 1): binomial checkpoints, each reversed
- * step recomputing its two stage states.  -1 stores every step together with its two stage
+ * step recomputing its two stage states (for nsteps >= 2 on a second stream, one reversal
+ * ahead of the adjoint kernels, in 9 more arrays of working memory when they can be
+ * allocated).  -1 stores every step together with its two stage
  * states (the backward then recomputes nothing; 9 arrays per step).  0 picks the storage
  * plan of swe_hip_adj_create_budget for 60% of the free device memory. */
 int swe_hip_adj_create(swe_hip_ctx *ctx, int64_t nsteps, int32_t nsnap, swe_hip_adj **out);
