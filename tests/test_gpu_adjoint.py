@@ -178,6 +178,23 @@ def test_gradient_independent_of_checkpoints(hip):
     assert any(k and -(-(9 - k) // m) >= 2 for k, m in plans)
 
 
+def test_binomial_overlap_long_schedule(hip):
+    """The binomial backward issues its advances and stage states on a second stream, one
+    reversal ahead, with double-buffered stage states and work state and waits on the last
+    reader of every slot: over 40 steps with 2, 4 and 7 slots the schedule reuses slots and
+    alternates the work buffers many times; the gradient must be the bits of the
+    store-every-step plan (a different executor on one stream)."""
+    case = mild_state(30, 64, 13)
+    _, _, dt = setup(case, 0)
+    rng = np.random.default_rng(5)
+    lam = [interior(case["h"].shape, rng) for _ in range(3)]
+    _, ref = gpu_gradient(hip, case, 0, 40, dt, lam, nsnap=-1)
+    for nsnap in (2, 4, 7):
+        _, g = gpu_gradient(hip, case, 0, 40, dt, lam, nsnap=nsnap)
+        for x, y in zip(g, ref):
+            assert same(x, y), nsnap
+
+
 @pytest.mark.parametrize("bc", [0, 1])
 def test_gradient_independent_of_launch_shape(hip, bc):
     """The fused kernel (x faces through LDS, blocks overlapping by 3 faces) and the
