@@ -210,8 +210,8 @@ int swe_hip_get_step_fusion(const swe_hip_ctx *ctx);
 /* Use a caller stream (hipStream_t passed as void*; NULL = the context's own stream). */
 int swe_hip_set_stream(swe_hip_ctx *ctx, void *stream);
 
-/* Replay-mode switch: 1 = capture one step in a hipGraph and replay it (default),
- * 0 = launch kernels directly. */
+/* Replay-mode switch: 1 = capture 32 steps in a hipGraph and replay it for every 32 steps of
+ * a call, launching the rest directly (default), 0 = launch kernels directly. */
 int swe_hip_set_graph(swe_hip_ctx *ctx, int enabled);
 
 /* Timing helper for benchmarks: runs nsteps fixed-dt steps with direct launches and HIP

@@ -138,6 +138,7 @@ def test_graph_and_direct_launches_agree(hip, bathy, recon, riemann):
             s.set_graph(graph)
             s.upload(h, hu, hv, B)
             s.step(dt, 9)
+            s.step(dt, 40)   # one replay of the 32-step graph and 8 direct steps
             outs.append(s.download())
     for a, b in zip(*outs):
         assert same(a, b)

@@ -347,9 +347,21 @@ def test_graph_and_direct_launch_agree(hip):
             dt, _ = s.stable_dt(0.4)
             s.step(dt, 5)
             s.step(dt, 6)
+            s.step(dt, 70)   # two replays of the 32-step graph and 6 direct steps
+            s.step(dt, 32)
             outs.append(s.download())
     for a, b in zip(*outs):
         assert_bitwise(a, b, "graph vs direct")
+    # adaptive steps: the dt history is written by the captured steps
+    outs, hists = [], []
+    for graph in (True, False):
+        with gpu_solver(hip, c) as s:
+            s.set_graph(graph)
+            hists.append(np.concatenate([s.step_adaptive(0.4, 40), s.step_adaptive(0.4, 70)]))
+            outs.append(s.download())
+    assert_bitwise(hists[0], hists[1], "adaptive dt history, graph vs direct")
+    for a, b in zip(*outs):
+        assert_bitwise(a, b, "adaptive, graph vs direct")
 
 
 @pytest.mark.parametrize("g", [9.80665, 9.81])
