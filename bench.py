@@ -223,7 +223,7 @@ def adjoint_bench(n=1024, steps=2000, plans=None):
     with swe_hip.HipSolver.from_case(case) as s:
         s.upload(case["h"], case["hu"], case["hv"], case["B"])
         dt, _ = s.stable_dt(0.4)
-        s.step(dt, 20)
+        s.step(dt, 20)   # warmup (also captures the step driver's graph)
         s.upload(case["h"], case["hu"], case["hv"], case["B"])
         s.sync()
         t0 = time.perf_counter()
@@ -431,7 +431,8 @@ def main():
     ap.add_argument("--event-steps", type=int, default=20)
     ap.add_argument("--no-ensemble", action="store_true")
     ap.add_argument("--ensemble-steps", type=int, default=500)
-    ap.add_argument("--ensemble-base-steps", type=int, default=30)
+    ap.add_argument("--ensemble-base-steps", type=int, default=32,
+                    help="per-instance baseline steps (32: one replay of the step driver's graph)")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-serial", action="store_true")
     ap.add_argument("--no-fused", action="store_true")

@@ -171,12 +171,13 @@ int run_fused_steps(swe_hip_ctx *c, int mode, double cfl, int64_t nsteps) {
 // one-step graph replayed per step ran a 1024^2 step (85 us of kernels) 5.6% and a 256^2
 // step 11% slower than direct launches, while 16- and 64-step graphs ran them 0.3-0.6% and
 // 3-4% faster than direct launches (8192^2: all the same; interleaved runs, one box).
-// Shorter runs, and the remainder of a long one, launch directly.
+// Steps beyond the last whole graph launch directly; any call of 2 steps or more captures
+// the graph if it is missing, so a short warm-up call prepares the long ones.
 constexpr int kGraphSteps = 32;
 
 int run_steps(swe_hip_ctx *c, int mode, double cfl, int64_t nsteps) {
     if (fused_mode(c)) return run_fused_steps(c, mode, cfl, nsteps);
-    if (!c->use_graph || nsteps < kGraphSteps) {
+    if (!c->use_graph || nsteps < 2) {
         for (int64_t s = 0; s < nsteps; ++s) {
             const int rc = enqueue_step(c, mode, cfl);
             if (rc) return rc;
