@@ -46,11 +46,17 @@ thread_local std::string g_create_err;
 
 int default_tile_rows(const GridDev &G) {
     // 32 rows per wave measured best on 8192^2 (vs 64/128/256: fewer waves idle at the end
-    // of the launch); smaller grids halve it down to 4 to keep ~3 waves per SIMD busy
-    // (256 CUs x 4 SIMDs x 3 waves), trading the 4-row prologue of each segment.
+    // of the launch); smaller grids halve it, down to 4, until a launch holds two rounds of
+    // the 3072 resident waves (256 CUs x 4 SIMDs x 3), trading the 4-row prologue of each
+    // segment.  One round was the bar before the prologue's rows went out together: two
+    // took 2048^2 from 16 to 8 rows (-1.8% per step) and 2560^2 from 32 / 24 to 16 rows
+    // (-3.2%), 3072^2 to 16 rows (-0.6%) and 1536^2 from 8 to 4 rows (-1.3%); 256^2 -
+    // 1024^2 and 4096^2 - 8192^2 unchanged (interleaved runs, one box).  The optimum is not
+    // monotone in the size (1536^2 runs 16-row tiles 3.6% faster still), so callers that
+    // step one size for long can time a few heights (swe_hip_set_stage_tile_rows).
     const long strips = swe::strips_for(G.ny);
     int tr = 32;
-    while (tr > 4 && strips * ((G.nx + tr - 1) / tr) < 3L * 256 * 4) tr /= 2;
+    while (tr > 4 && strips * ((G.nx + tr - 1) / tr) < 2L * 3 * 256 * 4) tr /= 2;
     return tr;
 }
 
